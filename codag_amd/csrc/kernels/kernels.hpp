@@ -1,0 +1,37 @@
+// kernels.hpp -- host-callable launchers of the gfx950 decode kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "codag.h"
+
+namespace codag {
+
+constexpr int kWaveSize = 64;
+
+// RLE launch shapes: (threads per group, LDS window) pairs that are compiled in.
+bool rle_shape_supported(uint32_t nt, uint32_t stage);
+
+hipError_t launch_rlev1_decode(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
+                               const uint64_t *col_len, uint64_t n_chunks, uint32_t w,
+                               uint64_t chunk_bytes, uint8_t *out, int32_t *status,
+                               uint32_t nt, uint32_t stage, hipStream_t stream);
+
+hipError_t launch_rlev2_decode(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
+                               const uint64_t *col_len, uint64_t n_chunks, uint32_t w,
+                               uint64_t chunk_bytes, uint8_t *out, int32_t *status,
+                               uint32_t nt, uint32_t stage, hipStream_t stream);
+
+hipError_t launch_inflate_chunks(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *off,
+                                 uint64_t n,
+                                 uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,
+                                 int32_t *status, uint32_t flags, hipStream_t stream);
+
+// Persistent batch launch: `groups` resident workgroups pull streams from
+// *counter (zeroed by a kernel launched first on the same stream).
+hipError_t launch_inflate_batch(const codag_stream_desc *descs, uint64_t n,
+                                codag_stream_result *results, uint32_t flags, uint32_t *counter,
+                                uint32_t groups, hipStream_t stream);
+
+}  // namespace codag

@@ -1,0 +1,338 @@
+/*
+ * codag.h -- C ABI of the MI355X-native CODAG decompressors (libcodag.so).
+ *
+ * Plain C: pointers, sizes and status codes only. Device pointers ("d_*") are
+ * HIP device allocations (hipMalloc, or codag_dev_alloc below); a
+ * codag_stream_t is a hipStream_t passed as void* (NULL = the context's own
+ * stream). Every call returns a codag_status and prints nothing; decode calls
+ * are stream-ordered and asynchronous unless the name ends in _host.
+ *
+ * Reference interfaces replaced (paths relative to illinois-impact/codag):
+ *   rle_v1::decompress_gpu_orig   include/rlev1/rlev1.h:419-424
+ *   rle_v1::compress_gpu_orig     include/rlev1/rlev1.h:103-192
+ *   rlev2::decompress_gpu_orig    include/rlev2/decoder_trans.h:57-60
+ *   rlev2::compress_gpu_orig      include/rlev2/encoder_trans.h:482-564
+ *   deflate::decompress_gpu       include/zlib/zlib.h:1585-1587
+ *   deflate_compressor.py         zlib_chunk_compress, deflate_compressor.py:44-126
+ *   cuda_err_chk / PRINT_ERROR    include/common.h:50-64 (print-and-continue
+ *                                 becomes a returned status)
+ * The batch descriptor mirrors the shape of cuDF's gpu_inflate_input_s /
+ * gpu_inflate_status_s (include/cudf_deflate.cuh:236-250).
+ */
+#ifndef CODAG_H
+#define CODAG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef enum codag_status {
+  CODAG_OK = 0,
+  CODAG_ERR_INVALID = 1,     /* bad argument (null pointer, unsupported width, ...) */
+  CODAG_ERR_HIP = 2,         /* a HIP runtime call failed */
+  CODAG_ERR_NOMEM = 3,       /* host or device allocation failed */
+  CODAG_ERR_IO = 4,          /* file could not be opened / read / written */
+  CODAG_ERR_FORMAT = 5,      /* container or side file is inconsistent */
+  CODAG_ERR_DECODE = 6,      /* at least one chunk/stream has a non-OK chunk status */
+  CODAG_ERR_BUSY = 7,        /* server admission: budget or slots exhausted */
+  CODAG_ERR_UNSUPPORTED = 8, /* feature not available (e.g. no GPU) */
+  CODAG_ERR_TIMEOUT = 9,     /* a wait exceeded its deadline */
+} codag_status;
+
+/* Per-chunk / per-stream codes written by the decode kernels. */
+enum {
+  CODAG_CHUNK_OK = 0,
+  CODAG_CHUNK_TRUNCATED = 1, /* stream ended before the chunk/block was complete */
+  CODAG_CHUNK_OVERFLOW = 2,  /* decoded data exceeds the output capacity */
+  CODAG_CHUNK_CORRUPT = 3,   /* invalid field, code, distance, header or checksum */
+};
+
+const char *codag_version(void);
+const char *codag_status_string(codag_status s);
+/* Message of the last failing call on this host thread ("" if none). */
+const char *codag_last_error(void);
+
+/* ----------------------------------------------------------------- context */
+/* A context binds one device, one HIP stream (default priority), the tuned
+ * launch parameters and grow-only scratch memory. One context per host
+ * thread, or external locking; different contexts are independent. */
+typedef struct codag_ctx codag_ctx;
+typedef void *codag_stream_t;
+
+codag_status codag_ctx_create(int device, codag_ctx **out);
+codag_status codag_ctx_destroy(codag_ctx *ctx);
+codag_stream_t codag_ctx_stream(codag_ctx *ctx);
+/* Number of visible devices (0 when no GPU / no driver). Never fails. */
+int codag_device_count(void);
+
+/* ------------------------------------------------------------------ RLE v1 */
+/* Replaces rle_v1::decompress_gpu_orig (include/rlev1/rlev1.h:419-592).
+ * d_comp   : chunk streams (the container minus its 8-byte length word,
+ *            rlev1.h:521 passes d_in + 8)
+ * d_blk_off: n_chunks byte offsets of the streams inside d_comp
+ * d_col_len: n_chunks stream lengths in bytes
+ * Each chunk decodes exactly chunk_bytes/elem_bytes values (decoder.h:671)
+ * into d_out + i*chunk_bytes. elem_bytes in {1,2,4,8}.
+ * d_chunk_status (nullable) receives one CODAG_CHUNK_* per chunk.
+ * comp_bytes bounds every stream: a chunk whose blk_off+col_len exceeds it is
+ * reported CORRUPT without being read (the reference trusts the side files). */
+codag_status codag_rlev1_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
+                                const uint64_t *d_blk_off, const uint64_t *d_col_len,
+                                uint64_t n_chunks, uint32_t elem_bytes,
+                                uint64_t chunk_bytes, uint8_t *d_out, int32_t *d_chunk_status,
+                                codag_stream_t stream);
+
+/* ------------------------------------------------------------------ RLE v2 */
+/* Replaces rlev2::decompress_gpu_orig (include/rlev2/decoder_trans.h:57-133).
+ * Chunk i's stream is d_comp[d_blk_off[i] .. +d_col_len[i]) and decodes into
+ * d_out + i*chunk_bytes; positions a chunk does not produce are zero-filled
+ * (the reference memsets the output first, decoder_trans.h:81). */
+codag_status codag_rlev2_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
+                                const uint64_t *d_blk_off, const uint64_t *d_col_len,
+                                uint64_t n_chunks, uint32_t elem_bytes,
+                                uint64_t chunk_bytes, uint8_t *d_out, int32_t *d_chunk_status,
+                                codag_stream_t stream);
+
+/* ----------------------------------------------------------------- Deflate */
+enum {
+  CODAG_INFLATE_SKIP_ADLER = 1u << 0,   /* do not compare the Adler-32 trailer (still required) */
+  CODAG_INFLATE_RAW = 1u << 1,          /* streams are raw deflate (no zlib header/trailer) */
+};
+
+/* Replaces deflate::decompress_gpu (include/zlib/zlib.h:1585-1700).
+ * d_offsets: n_chunks+1 byte offsets into d_comp (the blk_offset file minus
+ * its leading length word, zlib.h:1603); stream i is [off[i], off[i+1]) and
+ * inflates into d_out + i*chunk_bytes (at most chunk_bytes).
+ * d_out_len (nullable): bytes produced per chunk. The zlib header is checked
+ * as zlib does (the reference skips it unchecked, zlib.h:819). */
+codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
+                                  const uint64_t *d_offsets, uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *d_out,
+                                  uint64_t *d_out_len, int32_t *d_chunk_status, uint32_t flags,
+                                  codag_stream_t stream);
+
+/* Uneven batch of independent zlib streams (no reference counterpart: the
+ * reference has fixed-size chunks only; shape follows cudf_deflate.cuh:236-250).
+ * One persistent launch pulls streams from a device work queue, largest first. */
+typedef struct codag_stream_desc {
+  const uint8_t *src; /* device */
+  uint64_t src_bytes;
+  uint8_t *dst; /* device */
+  uint64_t dst_cap;
+} codag_stream_desc;
+
+typedef struct codag_stream_result {
+  uint64_t bytes_written;
+  int32_t status; /* CODAG_CHUNK_* */
+  uint32_t adler32;  /* Adler-32 of the bytes written */
+} codag_stream_result;
+
+codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_descs, uint64_t n,
+                                 codag_stream_result *d_results, uint32_t flags,
+                                 codag_stream_t stream);
+
+/* ------------------------------------------------- host convenience (sync) */
+/* Used by the drop-in CLIs. Inputs are host buffers laid out exactly as the
+ * reference's files; outputs are host buffers of at least the original
+ * length. kernel_seconds (nullable) gets the decode kernel time measured with
+ * HIP events; total_seconds (nullable) includes the host<->device copies.
+ *
+ * RLE v1: container = [u64 original_len][streams] (rlev1.h:163,175);
+ * blk_off_file = rle_v1_blk_offset.bin (num_chunks*32+1 u64, rlev1.h:469);
+ * col_len_file = rle_v1_col_len.bin (>= num_chunks u64). */
+codag_status codag_rlev1_decode_host(codag_ctx *ctx, const uint8_t *container,
+                                     uint64_t container_bytes, const uint64_t *blk_off_file,
+                                     uint64_t blk_off_entries, const uint64_t *col_len_file,
+                                     uint64_t col_len_entries, uint32_t elem_bytes,
+                                     uint64_t chunk_bytes, uint8_t *out, uint64_t out_cap,
+                                     uint64_t *out_bytes, double *kernel_seconds,
+                                     double *total_seconds);
+
+/* RLE v2: comp = concatenated chunk streams; blk_off has n_chunks+1 entries
+ * with blk_off[n_chunks] = original byte length (encoder_trans.h:545);
+ * col_len has >= n_chunks entries. */
+codag_status codag_rlev2_decode_host(codag_ctx *ctx, const uint8_t *comp, uint64_t comp_bytes,
+                                     const uint64_t *blk_off, const uint64_t *col_len,
+                                     uint64_t n_chunks, uint32_t elem_bytes, uint64_t chunk_bytes,
+                                     uint8_t *out, uint64_t out_cap, uint64_t *out_bytes,
+                                     double *kernel_seconds, double *total_seconds);
+
+/* Deflate: blk_offset_file = [u64 original_len][0][cumulative offsets...]
+ * (n_chunks+2 entries, deflate_compressor.py:59-60,118-124; zlib.h:1589). */
+codag_status codag_inflate_chunks_host(codag_ctx *ctx, const uint8_t *comp, uint64_t comp_bytes,
+                                       const uint64_t *blk_offset_file, uint64_t blk_entries,
+                                       uint64_t chunk_bytes, uint32_t flags, uint8_t *out,
+                                       uint64_t out_cap, uint64_t *out_bytes,
+                                       double *kernel_seconds, double *total_seconds);
+
+/* --------------------------------------------------- producers (CPU, C++) */
+/* Buffers returned by producers are allocated by the library; release them
+ * with codag_host_free. */
+void codag_host_free(void *p);
+
+/* RLE v1 container writer (role of rle_v1::compress_gpu_orig, rlev1.h:103-192):
+ * pads the input to a multiple of 32 chunks with zeros (rlev1.h:111-127),
+ * encodes each chunk, pads each stream to 4 bytes (encoder.h:780-783).
+ * Outputs: container [u64 in_bytes][streams]; blk_off (n_chunks*32+1 u64, the
+ * first n_chunks+1 meaningful, the rest repeat the total); col_len
+ * (n_chunks*32*32 u64, the first n_chunks meaningful, the rest zero) -- the
+ * sizes the reference writes to ./input_data/ (rlev1.h:178-186).
+ * Byte-identity with the reference encoder's choices is not claimed. */
+codag_status codag_rlev1_encode(const uint8_t *in, uint64_t in_bytes, uint32_t elem_bytes,
+                                uint64_t chunk_bytes, uint8_t **container,
+                                uint64_t *container_bytes, uint64_t **blk_off,
+                                uint64_t *blk_off_entries, uint64_t **col_len,
+                                uint64_t *col_len_entries, uint64_t *n_chunks);
+
+/* RLE v2 container writer (role of rlev2::compress_gpu_orig, encoder_trans.h:
+ * 482-564): n_chunks rounded up to a multiple of 32, streams aligned to 128 B,
+ * blk_off has n_chunks+1 entries with the last = in_bytes, col_len n_chunks. */
+codag_status codag_rlev2_encode(const uint8_t *in, uint64_t in_bytes, uint32_t elem_bytes,
+                                uint64_t chunk_bytes, uint8_t **comp, uint64_t *comp_bytes,
+                                uint64_t **blk_off, uint64_t **col_len, uint64_t *n_chunks);
+
+/* Deflate container writer (restates zlib_chunk_compress,
+ * deflate_compressor.py:44-126, with a selectable level): zero-pads the input
+ * to whole chunks, compresses each chunk as a zlib stream, pads each stream
+ * to 8 B. blk_offset = [in_bytes][0][cumulative...] (n_chunks+2 entries),
+ * col_len = padded sizes (n_chunks entries). */
+codag_status codag_deflate_compress_chunks(const uint8_t *in, uint64_t in_bytes,
+                                           uint64_t chunk_bytes, int level, int threads,
+                                           uint8_t **comp, uint64_t *comp_bytes,
+                                           uint64_t **blk_offset, uint64_t **col_len,
+                                           uint64_t *n_chunks);
+
+/* ------------------------------------------ synthetic workloads (CPU, C++) */
+/* Seeded generators of the benchmark columns/streams. Each returns the raw
+ * data and the encoded form ready for decode. */
+
+/* RLE v1 column: alternating runs (length min(130, 3+Geometric(mean 5)),
+ * 30% with a nonzero delta in [-128,127]) and literal groups of 1-128 values,
+ * values uniform in [0, value_range); segments never cross chunks. */
+codag_status codag_synth_rlev1(uint64_t n_values, uint32_t elem_bytes, uint64_t chunk_bytes,
+                               uint64_t value_range, uint64_t seed, uint8_t **raw,
+                               uint8_t **container, uint64_t *container_bytes, uint64_t **blk_off,
+                               uint64_t **col_len, uint64_t *n_chunks);
+
+/* RLE v2 column: a 25/25/25/25 mix (by value count) of SHORT_REPEAT (runs
+ * 3-10), DIRECT (widths of the decode map up to the column width), PATCHED_BASE
+ * (5% outliers, gaps <= 255) and DELTA (monotone, deltas in [0,31], first >= 1);
+ * blocks never cross chunks; streams aligned to 128 B. blk_off has
+ * n_chunks+1 entries, the last = raw bytes. */
+codag_status codag_synth_rlev2(uint64_t n_values, uint32_t elem_bytes, uint64_t chunk_bytes,
+                               uint64_t seed, uint8_t **raw, uint8_t **comp,
+                               uint64_t *comp_bytes, uint64_t **blk_off, uint64_t **col_len,
+                               uint64_t *n_chunks);
+
+/* Deflate payload generator: 50% text (order-2 Markov over 64 symbols with
+ * repeats of length 3-258 at distances up to 32 KiB), 50% little-endian int32
+ * drawn from Zipf(1.1), alternating per `part_bytes` region. */
+codag_status codag_synth_deflate_payload(uint64_t n_bytes, uint64_t part_bytes, uint64_t seed,
+                                         uint8_t **raw);
+
+/* One zlib stream whose deflate blocks follow a fixed/dynamic/stored mix
+ * (percent, by block count), switching with deflateParams at block
+ * boundaries of about block_bytes input bytes. Returns the stream and the
+ * achieved block counts. */
+codag_status codag_deflate_compress_mixed(const uint8_t *in, uint64_t in_bytes,
+                                          uint32_t pct_fixed, uint32_t pct_stored,
+                                          uint64_t block_bytes, uint64_t seed, uint8_t **stream,
+                                          uint64_t *stream_bytes, uint64_t counts[3]);
+
+/* ---------------------------------------------------------------- autotune */
+typedef enum codag_codec { CODAG_RLEV1 = 1, CODAG_RLEV2 = 2, CODAG_DEFLATE = 3 } codag_codec;
+
+typedef struct codag_tune_params {
+  uint32_t threads_per_group; /* workgroup size (64..1024, multiple of 64) */
+  uint32_t window_bytes;      /* compressed bytes staged in LDS per window */
+  uint32_t groups_per_cu;     /* persistent launches: resident groups per CU */
+  uint32_t queue_depth;       /* persistent launches: work items claimed per grab */
+} codag_tune_params;
+
+codag_status codag_tune_get(codag_ctx *ctx, codag_codec codec, codag_tune_params *out);
+codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_params *p);
+/* Time each candidate on the given device-resident sample (same arguments as
+ * the matching decode call; for DEFLATE d_blk_off are the n+1 offsets and
+ * d_col_len is ignored) and keep the fastest whose output equals the first
+ * candidate's. best_ms (nullable) gets its time. */
+codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_comp,
+                            uint64_t comp_bytes, const uint64_t *d_blk_off, const uint64_t *d_col_len,
+                            uint64_t n_chunks, uint32_t elem_bytes, uint64_t chunk_bytes,
+                            uint8_t *d_out, int reps, codag_tune_params *best, float *best_ms);
+
+/* ---------------------------------------------------------------- serving */
+/* Continuous-batching decode server (no reference counterpart). A host
+ * scheduler thread admits requests against an HBM byte budget and a slot
+ * limit, copies inputs to device-resident buffers, launches each admitted
+ * request's decode on its own HIP stream so work overlaps on the device, and
+ * reports per-request latency. */
+typedef struct codag_server codag_server;
+
+typedef struct codag_server_config {
+  int device;
+  uint64_t hbm_budget_bytes; /* admission budget for device-resident request data */
+  uint32_t max_inflight;     /* concurrent requests on the device (HIP streams) */
+} codag_server_config;
+
+typedef struct codag_request {
+  codag_codec codec;
+  uint32_t elem_bytes;  /* RLE only */
+  uint64_t chunk_bytes; /* bytes of output per chunk */
+  uint64_t n_chunks;
+  const uint8_t *comp;  /* host: chunk streams (RLE v1: without the 8-byte word) */
+  uint64_t comp_bytes;
+  const uint64_t *blk_off; /* host: n_chunks entries (DEFLATE: n_chunks+1 offsets) */
+  const uint64_t *col_len; /* host: n_chunks entries (DEFLATE: unused) */
+  uint8_t *out;            /* host: receives n_chunks*chunk_bytes bytes */
+  uint64_t user_tag;
+} codag_request;
+
+typedef struct codag_completion {
+  uint64_t request_id;
+  uint64_t user_tag;
+  codag_status status;
+  uint32_t failed_chunks;
+  double queue_seconds;   /* submit -> admitted */
+  double latency_seconds; /* submit -> result in host memory */
+} codag_completion;
+
+codag_status codag_server_create(const codag_server_config *cfg, codag_server **out);
+/* Non-blocking. CODAG_ERR_BUSY never happens here: requests wait in a FIFO
+ * until admitted; a request larger than the whole budget is rejected with
+ * CODAG_ERR_INVALID. */
+codag_status codag_server_submit(codag_server *s, const codag_request *req, uint64_t *request_id);
+/* Wait up to timeout_ms (-1 = forever) for at least one completion; returns up
+ * to max completions. */
+codag_status codag_server_poll(codag_server *s, codag_completion *out, uint32_t max,
+                               uint32_t *n_out, int timeout_ms);
+/* Device bytes currently admitted, requests in flight, requests queued. */
+codag_status codag_server_stats(codag_server *s, uint64_t *admitted_bytes, uint32_t *inflight,
+                                uint32_t *queued);
+codag_status codag_server_destroy(codag_server *s);
+
+/* ------------------------------------------------- device plumbing helpers */
+/* Thin wrappers so hosts without their own HIP bindings (the ctypes tests,
+ * bench.py) can move data; no reference counterpart. */
+codag_status codag_dev_alloc(codag_ctx *ctx, uint64_t bytes, void **d_ptr);
+codag_status codag_dev_free(codag_ctx *ctx, void *d_ptr);
+codag_status codag_copy_h2d(codag_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
+codag_status codag_copy_d2h(codag_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
+codag_status codag_memset_d(codag_ctx *ctx, void *d_dst, int value, uint64_t bytes);
+codag_status codag_sync(codag_ctx *ctx);
+/* HIP events recorded on the context stream around a region. */
+codag_status codag_timer_start(codag_ctx *ctx);
+codag_status codag_timer_stop(codag_ctx *ctx, float *elapsed_ms);
+/* 256 numbered events per context for timing individual launches on a
+ * stream (NULL = the context stream); elapsed waits for event b. */
+codag_status codag_event_record(codag_ctx *ctx, uint32_t slot, codag_stream_t stream);
+codag_status codag_event_elapsed(codag_ctx *ctx, uint32_t a, uint32_t b, float *elapsed_ms);
+/* Device name / CU count of the context's device. */
+codag_status codag_device_info(codag_ctx *ctx, char *name, uint32_t name_cap, uint32_t *cu_count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CODAG_H */

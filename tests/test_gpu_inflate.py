@@ -1,0 +1,225 @@
+"""GPU parity of the Deflate decoder (codag_inflate_chunks / codag_inflate_batch)
+against zlib's inflate (the oracle): byte-identical output, same byte counts,
+same status codes, and the Adler-32 the kernel computes equals zlib's."""
+import zlib
+
+import numpy as np
+import pytest
+
+import codag_amd as ca
+import oracle
+from tests.gpu_helpers import inflate, say
+
+pytestmark = pytest.mark.gpu
+CH = 131072
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu_required):
+    c = ca.Context(0)
+    yield c
+    c.close()
+
+
+def container(chunks):
+    padded = [s + b"\0" * ((-len(s)) % 8) for s in chunks]
+    off = np.cumsum([0] + [len(p) for p in padded]).astype(np.uint64)
+    return np.frombuffer(b"".join(padded) or b"\0", np.uint8), off
+
+
+def check(ctx, streams, cap, flags=0):
+    comp, off = container(streams)
+    st, ln, out = inflate(ctx, comp, off, cap, flags)
+    for i, s in enumerate(streams):
+        rc, ref = oracle.zlib_inflate(s, cap)
+        assert st[i] == rc, (i, st[i], rc)
+        if rc == 0:
+            assert int(ln[i]) == len(ref)
+            assert out[i * cap: i * cap + len(ref)].tobytes() == ref.tobytes(), i
+    return st
+
+
+def z(data, level=6, strategy=zlib.Z_DEFAULT_STRATEGY, wbits=15):
+    c = zlib.compressobj(level, zlib.DEFLATED, wbits, 8, strategy)
+    return c.compress(data) + c.flush()
+
+
+def test_kats(ctx):
+    say("inflate KATs")
+    empty = bytes.fromhex("789c030000000001")
+    stored = bytes([0x78, 0x01, 0x01, 0x05, 0x00, 0xFA, 0xFF]) + b"hello" + \
+        (0x062C0215).to_bytes(4, "big")
+    st = check(ctx, [empty, stored], 64)
+    assert st.tolist() == [0, 0]
+
+
+@pytest.mark.parametrize("level,strategy", [(0, 0), (1, 0), (6, 0), (9, 0), (6, zlib.Z_FIXED),
+                                            (6, zlib.Z_HUFFMAN_ONLY), (6, zlib.Z_RLE),
+                                            (6, zlib.Z_FILTERED)])
+def test_levels_and_strategies(ctx, level, strategy):
+    say(f"inflate level={level} strategy={strategy}")
+    payload = ca.synth_deflate_payload(8 * CH, CH, seed=level * 10 + strategy)
+    rng = np.random.default_rng(level)
+    parts = [payload[i * CH:(i + 1) * CH].tobytes() for i in range(8)]
+    parts.append(rng.integers(0, 256, CH, dtype=np.uint8).tobytes())  # incompressible
+    parts.append(b"\0" * CH)                                           # one long run
+    parts.append(bytes(range(256)) * (CH // 256))                     # period-256 copies
+    parts.append(b"ab" * (CH // 2))                                    # dist-2 overlap
+    streams = [z(p, level, strategy) for p in parts]
+    st = check(ctx, streams, CH)
+    assert np.all(st == 0)
+
+
+def test_short_and_odd_sizes(ctx):
+    say("inflate odd sizes")
+    rng = np.random.default_rng(3)
+    streams = []
+    for n in [0, 1, 2, 3, 7, 63, 64, 65, 255, 256, 1000, 4095, 4097, 70000]:
+        streams.append(z(rng.integers(0, 6, n, dtype=np.uint8).tobytes(), 6))
+    check(ctx, streams, CH)
+
+
+def test_unaligned_stream_starts(ctx):
+    # streams packed without 8-byte padding start at arbitrary byte offsets
+    say("inflate unaligned")
+    rng = np.random.default_rng(4)
+    streams = [z(rng.integers(0, 20, int(rng.integers(100, 30000)), dtype=np.uint8).tobytes(), 6)
+               for _ in range(40)]
+    off = np.cumsum([0] + [len(s) for s in streams]).astype(np.uint64)
+    comp = np.frombuffer(b"".join(streams), np.uint8)
+    st, ln, out = inflate(ctx, comp, off, CH)
+    for i, s in enumerate(streams):
+        ref = zlib.decompress(s)
+        assert st[i] == 0 and int(ln[i]) == len(ref)
+        assert out[i * CH: i * CH + len(ref)].tobytes() == ref
+
+
+def test_raw_and_skip_adler_flags(ctx):
+    say("inflate flags")
+    data = ca.synth_deflate_payload(CH, CH, seed=5).tobytes()
+    raw = z(data, 6, wbits=-15)
+    comp, off = container([raw])
+    st, ln, out = inflate(ctx, comp, off, CH, ca.INFLATE_RAW)
+    assert st.tolist() == [0] and out[:len(data)].tobytes() == data
+    s = bytearray(z(data, 6))
+    s[-1] ^= 0xFF  # wrong Adler-32
+    comp, off = container([bytes(s)])
+    st, _, _ = inflate(ctx, comp, off, CH, 0)
+    assert st.tolist() == [3]
+    st, _, out = inflate(ctx, comp, off, CH, ca.INFLATE_SKIP_ADLER)
+    assert st.tolist() == [0] and out[:len(data)].tobytes() == data
+
+
+def test_corrupt_and_truncated(ctx):
+    say("inflate corrupt/truncated")
+    data = ca.synth_deflate_payload(CH, CH, seed=6).tobytes()
+    good = z(data, 6)
+    cases = [good[: len(good) // 2], good[:1], good[:-2]]
+    bad = bytearray(good)
+    bad[2] |= 0x06  # BTYPE 11
+    cases.append(bytes(bad))
+    hdr = bytearray(good)
+    hdr[1] ^= 1  # header check
+    cases.append(bytes(hdr))
+    st = check(ctx, cases, CH)
+    assert np.all(st != 0)
+
+
+def test_random_corruption_matches_zlib(ctx):
+    # flip bytes inside dynamic-block headers / data: status must match zlib's
+    say("inflate fuzz")
+    rng = np.random.default_rng(7)
+    data = ca.synth_deflate_payload(CH, CH, seed=7).tobytes()[:20000]
+    good = z(data, 6)
+    streams = []
+    for t in range(300):
+        s = bytearray(good)
+        for _ in range(int(rng.integers(1, 4))):
+            s[int(rng.integers(2, min(len(s), 200)))] ^= int(rng.integers(1, 256))
+        streams.append(bytes(s))
+    comp, off = container(streams)
+    st, ln, out = inflate(ctx, comp, off, CH)
+    agree = 0
+    for i, s in enumerate(streams):
+        rc, ref = oracle.zlib_inflate(s, CH)
+        if rc == 0:
+            assert st[i] == 0 and out[i * CH: i * CH + len(ref)].tobytes() == ref.tobytes()
+        else:
+            assert st[i] != 0, i
+        agree += int(st[i] == rc)
+    assert agree >= 0.9 * len(streams)
+
+
+def test_chunked_container_parity(ctx):
+    say("inflate container 64 chunks")
+    raw = ca.synth_deflate_payload(64 * CH - 999, CH, seed=8)
+    comp, blk, cl, n = ca.deflate_compress_chunks(raw, CH, level=6, threads=8)
+    st, ln, out = inflate(ctx, comp, blk[1:], CH)
+    assert np.all(st == 0) and np.all(ln == CH)
+    assert out[: len(raw)].tobytes() == raw.tobytes()
+    assert oracle.crc32(out[: len(raw)]) == oracle.crc32(raw)
+
+
+def test_host_wrapper_reference_shape(ctx):
+    say("inflate host wrapper")
+    raw = ca.synth_deflate_payload(5 * CH + 17, CH, seed=9)
+    comp, blk, cl, n = ca.deflate.compress_chunks(raw, CH, level=9)
+    out, secs = ca.deflate.decompress_gpu(ctx, comp, blk, CH)
+    assert out.tobytes() == raw.tobytes() and secs > 0
+
+
+def _batch(ctx, streams, caps, flags=0):
+    src = np.frombuffer(b"".join(streams) or b"\0", np.uint8)
+    soff = np.cumsum([0] + [len(s) for s in streams]).astype(np.uint64)
+    doff = np.cumsum([0] + list(caps)).astype(np.uint64)
+    d_src = ctx.upload(src)
+    d_dst = ctx.alloc(int(doff[-1]) or 1)
+    d_dst.fill(0)
+    descs = (ca.StreamDesc * len(streams))()
+    for i in range(len(streams)):
+        descs[i].src = d_src.offset(int(soff[i]))
+        descs[i].src_bytes = len(streams[i])
+        descs[i].dst = d_dst.offset(int(doff[i]))
+        descs[i].dst_cap = int(caps[i])
+    d_desc = ctx.upload(np.frombuffer(bytes(descs), np.uint8))
+    d_res = ctx.alloc(16 * len(streams))
+    d_res.fill(0x7F)
+    ctx.inflate_batch(d_desc, len(streams), d_res, flags)
+    ctx.sync()
+    res = d_res.download().view(np.uint8).reshape(len(streams), 16)
+    written = res[:, :8].copy().view(np.uint64).reshape(-1)
+    status = res[:, 8:12].copy().view(np.int32).reshape(-1)
+    adler = res[:, 12:16].copy().view(np.uint32).reshape(-1)
+    return status, written, adler, d_dst.download(), doff
+
+
+def test_batch_uneven_streams(ctx):
+    say("inflate batch uneven")
+    rng = np.random.default_rng(10)
+    sizes = np.exp(rng.uniform(np.log(4096), np.log(600000), 60)).astype(int)
+    payload = ca.synth_deflate_payload(int(sizes.sum()), CH, seed=10)
+    streams, raws, pos = [], [], 0
+    for i, n in enumerate(sizes):
+        r = payload[pos: pos + n].tobytes()
+        pos += n
+        raws.append(r)
+        s, counts = ca.deflate_compress_mixed(np.frombuffer(r, np.uint8), 20, 5, 32768, seed=i)
+        streams.append(s.tobytes())
+    caps = [len(r) for r in raws]
+    for rep in range(3):  # the second and third calls replay the captured graph
+        status, written, adler, dst, doff = _batch(ctx, streams, caps)
+        for i, r in enumerate(raws):
+            assert status[i] == 0 and int(written[i]) == len(r)
+            assert dst[int(doff[i]): int(doff[i]) + len(r)].tobytes() == r
+            assert int(adler[i]) == zlib.adler32(r)
+
+
+def test_batch_errors_and_overflow(ctx):
+    say("inflate batch errors")
+    data = ca.synth_deflate_payload(50000, CH, seed=11).tobytes()
+    good = z(data, 6)
+    streams = [good, good[:100], good, b"", good]
+    caps = [len(data), len(data), 1000, 16, len(data)]
+    status, written, adler, dst, doff = _batch(ctx, streams, caps)
+    exp = [oracle.zlib_inflate(s, c)[0] for s, c in zip(streams, caps)]
+    assert status.tolist() == exp

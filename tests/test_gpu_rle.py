@@ -1,0 +1,264 @@
+"""GPU parity of the RLE v1 / RLE v2 decode kernels against the CPU oracle.
+
+Every decode goes through the C ABI (codag_rlev1_decode / codag_rlev2_decode)
+on device-resident buffers and must be byte-identical to the oracle, with the
+same per-chunk status codes."""
+import numpy as np
+import pytest
+
+import codag_amd as ca
+import oracle
+from tests import pycodecs as pc
+from tests.gpu_helpers import rle_decode, say, single_chunk
+
+pytestmark = pytest.mark.gpu
+CH = 131072
+U64 = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu_required):
+    c = ca.Context(0)
+    yield c
+    c.close()
+
+
+def _check_chunks(ctx, codec, comp, blk, cl, width, chunk):
+    n = len(cl)
+    st, out = rle_decode(ctx, codec, comp, blk[:n], cl, width, chunk)
+    fn = oracle.rlev1_decode_chunks if codec == ca.RLEV1 else oracle.rlev2_decode_chunks
+    ost, oout = fn(comp, blk[:n], cl, width, chunk, threads=8)
+    assert st.tolist() == ost.tolist()
+    ok = ost == 0
+    o = out.reshape(n, chunk)[ok]
+    e = oout.reshape(n, chunk)[ok]
+    if not np.array_equal(o, e):
+        bad = np.nonzero((o != e).any(axis=1))[0]
+        raise AssertionError(f"{len(bad)} chunks differ, first {bad[:5]}")
+    return st, out
+
+
+# ------------------------------------------------------------------ RLE v1
+
+V1_KATS = [
+    (bytes([0x00, 0x01, 0x05]), 8, [5, 6, 7]),
+    (bytes([0xFE, 0x81, 0x01, 0x7F]), 8, [129, 127]),
+    (bytes([0x7F, 0xFF, 0x0A]), 8, [(10 - i) & U64 for i in range(130)]),
+    (bytes([0x80]) + b"".join(pc.uleb(x) for x in range(1000, 1128)), 8, list(range(1000, 1128))),
+    (bytes([0x00, 0x01]) + pc.uleb(300), 1, [44, 45, 46]),
+    (bytes([0xFF]) + pc.uleb(1 << 63), 8, [1 << 63]),
+]
+
+
+@pytest.mark.parametrize("k", range(len(V1_KATS)))
+def test_v1_kats(ctx, k):
+    stream, w, expect = V1_KATS[k]
+    say(f"rlev1 KAT {k}")
+    comp, blk, cl = single_chunk(stream)
+    st, out = rle_decode(ctx, ca.RLEV1, comp, blk, cl, w, len(expect) * w)
+    dt = {1: np.uint8, 8: np.uint64}[w]
+    assert st.tolist() == [0]
+    assert out.view(dt).tolist() == expect
+
+
+@pytest.mark.parametrize("width", [8, 4, 2, 1])
+def test_v1_synth_parity(ctx, width):
+    say(f"rlev1 synth width={width}")
+    nvals = 40 * CH // width + 333
+    raw, cont, blk, cl, n = ca.synth_rlev1(nvals, width, CH, 1 << 16, seed=11 + width)
+    st, out = _check_chunks(ctx, ca.RLEV1, cont[8:], blk, cl, width, CH)
+    assert np.all(st == 0)
+    assert out[: len(raw)].tobytes() == raw.tobytes()
+
+
+@pytest.mark.parametrize("width", [8, 4, 2, 1])
+def test_v1_encoder_parity(ctx, width):
+    say(f"rlev1 encoder width={width}")
+    rng = np.random.default_rng(width)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[width]
+    col = np.concatenate([rng.integers(0, 1 << 62, 20000).astype(dt),
+                          np.repeat(rng.integers(0, 255, 3000), 50).astype(dt),
+                          (np.arange(90000) * 3).astype(dt)])
+    cont, blk, cl, n = ca.rlev1_encode(col, width)
+    st, out = _check_chunks(ctx, ca.RLEV1, cont[8:], blk, cl[:n], width, CH)
+    assert out[: col.nbytes].tobytes() == col.tobytes()
+
+
+def test_v1_many_small_chunks_and_max_segments(ctx):
+    # chunks of 64 values: runs of max length 130 overflow, literal groups of 128 do not
+    say("rlev1 small chunks")
+    rng = np.random.default_rng(5)
+    streams, vals = [], []
+    for i in range(300):
+        if i % 3 == 0:
+            v = [int(x) for x in rng.integers(0, 1 << 40, 64)]
+            s = bytes([0xC0]) + b"".join(pc.uleb(x) for x in v)
+        elif i % 3 == 1:
+            v = [7 + j * 2 for j in range(64)]
+            s = bytes([61, 2]) + pc.uleb(7)
+        else:
+            v = [int(x) for x in rng.integers(0, 1 << 62, 64)]
+            s = pc.rlev1_encode(v, 8)
+        streams.append(s)
+        vals.append(v)
+    off = np.cumsum([0] + [len(s) for s in streams])[:-1].astype(np.uint64)
+    cl = np.array([len(s) for s in streams], np.uint64)
+    comp = np.frombuffer(b"".join(streams), np.uint8)
+    st, out = _check_chunks(ctx, ca.RLEV1, comp, off, cl, 8, 64 * 8)
+    assert np.all(st == 0)
+    assert out.view(np.uint64).tolist() == [x & U64 for v in vals for x in v]
+
+
+V1_BAD = [
+    (bytes([0x00, 0x01]), 24),                       # truncated run
+    (bytes([0x05, 0x00, 0x01]), 24),                 # run overflows the chunk
+    (b"", 8),                                        # empty stream
+    (bytes([0xFF]) + bytes([0x80] * 10) + bytes([1]), 8),  # 11-byte varint
+    (bytes([0xFE, 0x81]), 16),                       # truncated literal
+]
+
+
+@pytest.mark.parametrize("k", range(len(V1_BAD)))
+def test_v1_error_status_matches_oracle(ctx, k):
+    stream, chunk = V1_BAD[k]
+    say(f"rlev1 error case {k}")
+    comp, blk, cl = single_chunk(stream, pad=16)
+    st, _ = rle_decode(ctx, ca.RLEV1, comp, blk, cl, 8, chunk)
+    rc, _ = oracle.rlev1_decode_chunk(stream, 8, chunk)
+    assert rc != 0 and st.tolist() == [rc]
+
+
+def test_v1_stream_outside_buffer_is_corrupt(ctx):
+    say("rlev1 out-of-range blk_off")
+    comp = np.zeros(64, np.uint8)
+    st, _ = rle_decode(ctx, ca.RLEV1, comp, np.array([60], np.uint64),
+                       np.array([100], np.uint64), 8, 64)
+    assert st.tolist() == [3]
+
+
+# ------------------------------------------------------------------ RLE v2
+
+V2_KATS = [
+    ("0a2710", 8, [10000] * 5),
+    ("5e035ca1ab1edeadbeef", 4, [23713, 43806, 57005, 48879]),
+    ("c609020222424246", 8, [2, 3, 5, 7, 11, 13, 17, 19, 23, 29]),
+    ("8e132b2107d01e00147028323c46505a646e78828c96a0aab4befce8", 8,
+     [2030, 2000, 2020, 1000000] + list(range(2040, 2200, 10))),
+]
+
+
+@pytest.mark.parametrize("k", range(len(V2_KATS)))
+def test_v2_kats(ctx, k):
+    hexs, w, expect = V2_KATS[k]
+    say(f"rlev2 KAT {k}")
+    comp, blk, cl = single_chunk(bytes.fromhex(hexs), pad=16)
+    st, out = rle_decode(ctx, ca.RLEV2, comp, blk, cl, w, 64 * w)
+    dt = {4: np.uint32, 8: np.uint64}[w]
+    assert st.tolist() == [0]
+    v = out.view(dt)
+    assert v[: len(expect)].tolist() == expect and not v[len(expect):].any()
+
+
+def test_v2_semantic_corners(ctx):
+    say("rlev2 corners")
+    cases = [pc.v2_delta(100, 0, [1, 2, 3], code=3), pc.v2_delta(100, -1, [1, 2, 3], code=3),
+             pc.v2_delta(7, 5, [0] * 6, code=None), bytes([0x40 | (7 << 1), 0xFF]),
+             bytes([0xC0, 0x02]) + pc.uleb(5) + pc.uleb(U64),
+             pc.v2_short_repeat(0x0102030405, 3, 5)]
+    for w in (8, 4):
+        streams = cases
+        off = np.cumsum([0] + [len(s) for s in streams])[:-1].astype(np.uint64)
+        cl = np.array([len(s) for s in streams], np.uint64)
+        comp = np.frombuffer(b"".join(streams) + b"\0" * 16, np.uint8)
+        st, out = _check_chunks(ctx, ca.RLEV2, comp, off, cl, w, 512 * w)
+        assert np.all(st == 0)
+
+
+@pytest.mark.parametrize("code", range(32))
+def test_v2_every_width(ctx, code):
+    bits = pc.DECODE_WIDTHS[code]
+    rng = np.random.default_rng(code)
+    vals = [int(rng.integers(0, 1 << min(bits, 62))) << max(0, bits - 62) for _ in range(255)]
+    vals[0] = (1 << bits) - 1
+    s = pc.v2_direct(vals[:128], code) + pc.v2_direct(vals[128:], code)
+    comp, blk, cl = single_chunk(s, pad=16)
+    for w in (8, 4, 2, 1):
+        st, out = _check_chunks(ctx, ca.RLEV2, comp, blk, cl, w, 256 * w)
+        assert st.tolist() == [0]
+
+
+@pytest.mark.parametrize("width", [8, 4, 2, 1])
+def test_v2_synth_parity(ctx, width):
+    say(f"rlev2 synth width={width}")
+    nvals = 40 * CH // width + 77
+    raw, comp, blk, cl, n = ca.synth_rlev2(nvals, width, CH, seed=21 + width)
+    st, out = _check_chunks(ctx, ca.RLEV2, comp, blk, cl, width, CH)
+    assert np.all(st == 0)
+    assert out[: len(raw)].tobytes() == raw.tobytes()
+
+
+@pytest.mark.parametrize("width", [8, 4, 2, 1])
+def test_v2_encoder_parity(ctx, width):
+    say(f"rlev2 encoder width={width}")
+    rng = np.random.default_rng(40 + width)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[width]
+    col = np.concatenate([rng.integers(0, 1 << 62, 20000).astype(dt),
+                          np.repeat(rng.integers(0, 255, 3000), 7).astype(dt),
+                          (np.arange(90000) * 3 + 5).astype(dt),
+                          (rng.integers(0, 50, 30000) + (rng.random(30000) < 0.03) * 10**6).astype(dt)])
+    comp, blk, cl, n = ca.rlev2_encode(col, width)
+    st, out = _check_chunks(ctx, ca.RLEV2, comp, blk, cl, width, CH)
+    assert out[: col.nbytes].tobytes() == col.tobytes()
+
+
+def test_v2_patched_base_random(ctx):
+    say("rlev2 patched base random")
+    rng = np.random.default_rng(9)
+    streams = []
+    for t in range(200):
+        n = int(rng.integers(1, 129))
+        code = int(rng.choice([3, 7, 11, 15]))
+        fbw = pc.DECODE_WIDTHS[code]
+        reduced = [int(x) for x in rng.integers(0, 1 << fbw, n)]
+        pw_code = int(rng.choice([0, 3, 7, 11, 15, 20, 27]))
+        pw = pc.DECODE_WIDTHS[pw_code]
+        pgw = int(rng.integers(1, 9))
+        idx = sorted(rng.choice(n, min(n, int(rng.integers(0, 8))), replace=False).tolist())
+        patches, prev = [], 0
+        for i in idx:
+            if i - prev < (1 << pgw):
+                patches.append((i - prev, int(rng.integers(0, 1 << pw))))
+                prev = i
+        streams.append(pc.v2_patched_base(reduced, int(rng.integers(0, 1 << 30)), code, 4,
+                                          patches, pw_code, pgw))
+    off = np.cumsum([0] + [len(s) for s in streams])[:-1].astype(np.uint64)
+    cl = np.array([len(s) for s in streams], np.uint64)
+    comp = np.frombuffer(b"".join(streams) + b"\0" * 16, np.uint8)
+    st, _ = _check_chunks(ctx, ca.RLEV2, comp, off, cl, 8, 256 * 8)
+    assert np.all(st == 0)
+
+
+V2_BAD = [
+    ("5e035ca1", 64),          # DIRECT truncated
+    ("0a2710", 3),             # overflow
+    ("c6", 64),                # DELTA header truncated
+]
+
+
+@pytest.mark.parametrize("k", range(len(V2_BAD)))
+def test_v2_error_status_matches_oracle(ctx, k):
+    hexs, cap = V2_BAD[k]
+    say(f"rlev2 error case {k}")
+    s = bytes.fromhex(hexs)
+    comp, blk, cl = single_chunk(s, pad=16)
+    st, _ = rle_decode(ctx, ca.RLEV2, comp, blk, cl, 8, cap * 8)
+    rc, _ = oracle.rlev2_decode_chunk(s, 8, cap)
+    assert rc != 0 and st.tolist() == [rc]
+
+
+def test_v2_patch_outside_block_is_corrupt(ctx):
+    say("rlev2 bad patch")
+    s = pc.v2_patched_base([1, 2], 0, 3, 1, [(5, 1)], 3, 3)
+    comp, blk, cl = single_chunk(s, pad=16)
+    st, _ = rle_decode(ctx, ca.RLEV2, comp, blk, cl, 8, 64 * 8)
+    assert st.tolist() == [3]
