@@ -36,6 +36,8 @@ INFLATE_SKIP_ADLER = 1
 INFLATE_RAW = 2
 # codecs
 RLEV1, RLEV2, DEFLATE = 1, 2, 3
+# server request flags
+REQ_DEVICE_IO = 1
 
 CHUNK_BYTES = 131072
 
@@ -80,7 +82,7 @@ class ServerConfig(ctypes.Structure):
 
 
 class Request(ctypes.Structure):
-    _fields_ = [("codec", ctypes.c_int), ("elem_bytes", _u32), ("chunk_bytes", _u64),
+    _fields_ = [("codec", ctypes.c_int), ("flags", _u32), ("elem_bytes", _u32), ("chunk_bytes", _u64),
                 ("n_chunks", _u64), ("comp", _vp), ("comp_bytes", _u64), ("blk_off", _vp),
                 ("col_len", _vp), ("out", _vp), ("user_tag", _u64)]
 

@@ -451,14 +451,17 @@ rlev2_decode_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
         uint32_t gsum = wave_incl_scan(g);
         uint64_t add = (fbw >= nbits) ? 0 : (((ent & pmask) << fbw) & mask);
         if (lane < pll && gsum >= len8) bad = 1;
-        for (uint32_t i = lane; i < len8; i += kWave) {
-          uint64_t v = lds_bits_be(L.sbuf, vb0 + i * fbw, fbw) & mask;
+        // all lanes stay active around the cross-lane reads (a lane shuffle
+        // from an inactive lane returns 0)
+        for (uint32_t c = 0; c < len8; c += kWave) {
+          const uint32_t i = c + lane;
+          uint64_t v = i < len8 ? (lds_bits_be(L.sbuf, vb0 + i * fbw, fbw) & mask) : 0;
           for (uint32_t j = 0; j < pll; j++) {
             const uint32_t gj = __shfl(gsum, j, kWave);
             const uint64_t aj = __shfl(add, j, kWave);
             if (gj == i) v |= aj;
           }
-          store_elem(dst + (uint64_t)(o + i) * w, w, (v + base) & mask);
+          if (i < len8) store_elem(dst + (uint64_t)(o + i) * w, w, (v + base) & mask);
         }
       }
     }

@@ -80,6 +80,30 @@ static codag_status run_one(codag_server *s, hipStream_t st, Grow *b, const Pend
   const uint64_t n = r.n_chunks;
   const uint64_t out_bytes = n * r.chunk_bytes;
   const uint64_t noff = r.codec == CODAG_DEFLATE ? n + 1 : n;
+  if (r.flags & CODAG_REQ_DEVICE_IO) {  // data already in HBM: decode in place
+    hipError_t e = b[4].ensure(n * 4);
+    if (e != hipSuccess) return fail(CODAG_ERR_NOMEM, std::string("server: ") + hipGetErrorString(e));
+    int32_t *stat = static_cast<int32_t *>(b[4].p);
+    codag_status cs;
+    if (r.codec == CODAG_RLEV1)
+      cs = codag_rlev1_decode(s->ctx, r.comp, r.comp_bytes, r.blk_off, r.col_len, n, r.elem_bytes,
+                              r.chunk_bytes, r.out, stat, st);
+    else if (r.codec == CODAG_RLEV2)
+      cs = codag_rlev2_decode(s->ctx, r.comp, r.comp_bytes, r.blk_off, r.col_len, n, r.elem_bytes,
+                              r.chunk_bytes, r.out, stat, st);
+    else
+      cs = codag_inflate_chunks(s->ctx, r.comp, r.comp_bytes, r.blk_off, n, r.chunk_bytes, r.out,
+                                nullptr, stat, 0, st);
+    if (cs) return cs;
+    std::vector<int32_t> hs(n);
+    if (n) e = hipMemcpyAsync(hs.data(), stat, n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(CODAG_ERR_HIP, std::string("server: ") + hipGetErrorString(e));
+    uint32_t bad = 0;
+    for (auto v : hs) bad += (v != 0);
+    *failed = bad;
+    return bad ? CODAG_ERR_DECODE : CODAG_OK;
+  }
   hipError_t e = b[0].ensure(r.comp_bytes);
   if (e == hipSuccess) e = b[1].ensure(noff * 8);
   if (e == hipSuccess) e = b[2].ensure(n * 8);
@@ -186,8 +210,10 @@ codag_status codag_server_submit(codag_server *s, const codag_request *req, uint
     return fail(CODAG_ERR_INVALID, "server_submit: null buffer");
   if (req->codec != CODAG_DEFLATE && req->n_chunks && !req->col_len)
     return fail(CODAG_ERR_INVALID, "server_submit: null col_len");
-  // validate the metadata on the host so a malformed request never reaches the GPU
-  for (uint64_t i = 0; i < req->n_chunks; i++) {
+  // validate host metadata so a malformed request never reaches the GPU
+  // (device-resident metadata is bounded by comp_bytes inside the kernels)
+  const bool host_meta = !(req->flags & CODAG_REQ_DEVICE_IO);
+  for (uint64_t i = 0; host_meta && i < req->n_chunks; i++) {
     if (req->codec == CODAG_DEFLATE) {
       if (req->blk_off[i] > req->blk_off[i + 1] || req->blk_off[i + 1] > req->comp_bytes)
         return fail(CODAG_ERR_FORMAT, "server_submit: chunk outside the stream");

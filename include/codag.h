@@ -277,17 +277,26 @@ typedef struct codag_server_config {
   uint32_t max_inflight;     /* concurrent requests on the device (HIP streams) */
 } codag_server_config;
 
+enum {
+  /* comp/blk_off/col_len/out are device pointers: the request's data already
+   * lives in HBM and its output stays there (no host copies). */
+  CODAG_REQ_DEVICE_IO = 1u << 0,
+};
+
 typedef struct codag_request {
   codag_codec codec;
+  uint32_t flags;       /* CODAG_REQ_* */
   uint32_t elem_bytes;  /* RLE only */
   uint64_t chunk_bytes; /* bytes of output per chunk */
   uint64_t n_chunks;
-  const uint8_t *comp;  /* host: chunk streams (RLE v1: without the 8-byte word) */
+  const uint8_t *comp;  /* chunk streams (RLE v1: without the 8-byte word) */
   uint64_t comp_bytes;
-  const uint64_t *blk_off; /* host: n_chunks entries (DEFLATE: n_chunks+1 offsets) */
-  const uint64_t *col_len; /* host: n_chunks entries (DEFLATE: unused) */
-  uint8_t *out;            /* host: receives n_chunks*chunk_bytes bytes */
+  const uint64_t *blk_off; /* n_chunks entries (DEFLATE: n_chunks+1 offsets) */
+  const uint64_t *col_len; /* n_chunks entries (DEFLATE: unused) */
+  uint8_t *out;            /* receives n_chunks*chunk_bytes bytes */
   uint64_t user_tag;
+  /* host pointers unless flags has CODAG_REQ_DEVICE_IO; for device-resident
+   * requests the metadata is trusted (bounded by comp_bytes in the kernels) */
 } codag_request;
 
 typedef struct codag_completion {

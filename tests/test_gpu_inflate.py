@@ -30,7 +30,9 @@ def container(chunks):
 def check(ctx, streams, cap, flags=0):
     comp, off = container(streams)
     st, ln, out = inflate(ctx, comp, off, cap, flags)
-    for i, s in enumerate(streams):
+    for i in range(len(streams)):
+        # the kernel sees the 8-byte-padded stream, so zlib gets the same bytes
+        s = comp[int(off[i]): int(off[i + 1])].tobytes()
         rc, ref = oracle.zlib_inflate(s, cap)
         assert st[i] == rc, (i, st[i], rc)
         if rc == 0:

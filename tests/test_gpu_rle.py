@@ -231,6 +231,10 @@ def test_v2_patched_base_random(ctx):
                 prev = i
         streams.append(pc.v2_patched_base(reduced, int(rng.integers(0, 1 << 30)), code, 4,
                                           patches, pw_code, pgw))
+    # patches in the second half of a 128-value block (second lane pass)
+    red = [int(x) for x in rng.integers(0, 8, 128)]
+    streams.append(pc.v2_patched_base(red, 1000, 2, 2, [(70, 5), (57, 9)], 3, 8))
+    streams.append(pc.v2_patched_base(red, 7, 2, 1, [(0, 1), (64, 3), (63, 2)], 7, 8))
     off = np.cumsum([0] + [len(s) for s in streams])[:-1].astype(np.uint64)
     cl = np.array([len(s) for s in streams], np.uint64)
     comp = np.frombuffer(b"".join(streams) + b"\0" * 16, np.uint8)
