@@ -78,7 +78,7 @@ codag_status check_rle_args(uint32_t w, uint64_t chunk_bytes) {
 
 void default_tune(codag_ctx *c) {
   c->tune[CODAG_RLEV1] = {256, 4096, 0, 0};
-  c->tune[CODAG_RLEV2] = {256, 4096, 0, 0};
+  c->tune[CODAG_RLEV2] = {64, 4096, 0, 0};
   c->tune[CODAG_DEFLATE] = {64, 0, 16, 1};
   c->tune[0] = {0, 0, 0, 0};
 }
@@ -204,9 +204,13 @@ codag_status codag_rlev2_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t 
   if (n_chunks > 0x7fffffffull) return fail(CODAG_ERR_INVALID, "rlev2_decode: too many chunks");
   DeviceGuard g(ctx->device);
   const codag_tune_params &t = ctx->tune[CODAG_RLEV2];
-  HIP_TRY(launch_rlev2_decode(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
-                              d_out, d_status, t.threads_per_group, t.window_bytes,
-                              pick(ctx, stream)));
+  if (t.threads_per_group == 64)  // one self-contained wave per chunk
+    HIP_TRY(launch_rlev2_wave(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
+                              d_out, d_status, t.window_bytes, pick(ctx, stream)));
+  else
+    HIP_TRY(launch_rlev2_decode(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w,
+                                chunk_bytes, d_out, d_status, t.threads_per_group, t.window_bytes,
+                                pick(ctx, stream)));
   return CODAG_OK;
 }
 
@@ -435,7 +439,10 @@ codag_status codag_tune_get(codag_ctx *ctx, codag_codec codec, codag_tune_params
 codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_params *p) {
   if (!ctx || !p || codec < CODAG_RLEV1 || codec > CODAG_DEFLATE)
     return fail(CODAG_ERR_INVALID, "tune_set: bad argument");
-  if (codec != CODAG_DEFLATE && !rle_shape_supported(p->threads_per_group, p->window_bytes))
+  const bool wave_shape = codec == CODAG_RLEV2 && p->threads_per_group == 64 &&
+                          (p->window_bytes == 4096 || p->window_bytes == 8192);
+  if (codec != CODAG_DEFLATE && !wave_shape &&
+      !rle_shape_supported(p->threads_per_group, p->window_bytes))
     return fail(CODAG_ERR_INVALID, "tune_set: launch shape not compiled in");
   if (codec == CODAG_DEFLATE && (p->groups_per_cu == 0 || p->groups_per_cu > 64))
     return fail(CODAG_ERR_INVALID, "tune_set: groups_per_cu must be 1..64");
@@ -456,6 +463,10 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
   if (codec == CODAG_DEFLATE) {
     for (uint32_t per : {4u, 8u, 12u, 16u, 20u}) cands.push_back({64, 0, per, 1});
   } else {
+    if (codec == CODAG_RLEV2) {  // one-wave-per-chunk shapes first
+      cands.push_back({64, 4096, 0, 0});
+      cands.push_back({64, 8192, 0, 0});
+    }
     const uint32_t shapes[][2] = {{256, 4096}, {128, 4096}, {512, 4096}, {256, 8192}, {512, 8192}};
     for (auto &sh : shapes) cands.push_back({sh[0], sh[1], 0, 0});
   }

@@ -60,7 +60,10 @@ template <int NSYM> struct Huff {
   uint16_t sym[NSYM];
 };
 
+constexpr uint32_t kRingBytes = 4096;  // output ring (see the output side below)
+
 struct Lds {
+  alignas(16) uint8_t ring[kRingBytes];
   uint32_t lit[1u << kLitBits];
   uint32_t dist[1u << kDistBits];
   Huff<288> hl;
@@ -74,7 +77,7 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_re
 
 struct Bits {
   const uint32_t *w;  // 4-aligned address at or below the stream start
-  uint64_t limit;     // bit index (from w) one past the stream's last bit
+  uint32_t limit;     // bit index (from w) one past the stream's last bit
   uint32_t nw;        // dwords that hold stream bytes
   uint32_t next;      // next dword to append to buf
   uint64_t buf;
@@ -113,7 +116,7 @@ __device__ __forceinline__ void bits_drop(Bits &b, uint32_t n) {
   b.cnt -= n;
 }
 
-__device__ __forceinline__ uint64_t bits_pos(const Bits &b) { return (uint64_t)b.next * 32 - b.cnt; }
+__device__ __forceinline__ uint32_t bits_pos(const Bits &b) { return b.next * 32 - b.cnt; }
 
 __device__ __forceinline__ bool bits_past_end(const Bits &b) { return bits_pos(b) > b.limit; }
 
@@ -132,66 +135,200 @@ __device__ __forceinline__ void bits_seek(Bits &b, uint64_t lead_plus_at) {
 
 // ------------------------------------------------------------- output side
 
+// The most recent kRing output bytes live in an LDS ring. Byte x of the
+// stream sits at ring[(x + mis) & (kRing-1)], where mis = dst & 15, so a
+// 16-byte aligned output address is also 16-byte aligned in the ring.
+//  * literals are buffered one per lane and written to the ring in one
+//    64-lane LDS store;
+//  * matches with distance <= kNear copy ring -> ring (LDS only);
+//  * farther matches read bytes that were flushed to HBM long ago; they are
+//    queued as pieces of <= 64 bytes (lane j holds piece j) and resolved in
+//    groups of 16 pieces per memory round trip;
+//  * every kFlush bytes the ring is written to HBM with 16-byte stores, and
+//    Adler-32 partial sums (sum b, sum pos*b) are accumulated on the way.
+// A global load waits for all earlier stores of the wave (one in-order
+// counter), so the design keeps global reads rare and batched.
+constexpr uint32_t kRing = kRingBytes;
+constexpr uint32_t kRingMask = kRing - 1;
+constexpr uint32_t kFlush = 1024;
+constexpr uint32_t kNear = kRing - 258 - 64 - kFlush;  // ring copies never read overwritten bytes
+constexpr uint32_t kPieceGroup = 8;
+// Latency of the serial symbol loop is hidden by other waves on the SIMD:
+// keep register use low enough for this many waves per SIMD.
+constexpr int kMinWavesPerEu = 6;
+
+// Positions are 32-bit (streams and outputs below 2^31 bytes) so the
+// wave-uniform state fits in scalar registers without spilling.
 struct Out {
-  uint8_t *dst;
-  uint64_t cap;
-  uint64_t pos;     // bytes produced (including buffered literals)
-  uint64_t synced;  // every byte below this has been stored and retired
-  uint32_t litbuf;  // one pending literal byte per lane
-  uint32_t nlit;    // pending literals (uniform)
-  uint64_t s1, s2;  // per-lane Adler-32 partial sums: sum b, sum pos*b
+  uint8_t *dst;      // global output
+  uint8_t *ring;     // LDS ring
+  uint32_t cap;
+  uint32_t pos;      // bytes produced (including buffered literals)
+  uint32_t flushed;  // bytes [0, flushed) have been stored to dst
+  uint32_t retired;  // bytes [0, retired) are known to have reached memory
+  uint32_t mis;      // dst & 15
+  uint32_t litbuf;   // pending literal byte of this lane
+  uint32_t nlit;     // pending literals (uniform)
+  uint32_t np;       // queued far pieces (uniform)
+  uint32_t pdst, psrc, plen;  // piece `lane` (valid for lane < np)
+  uint32_t hole_lo, hole_hi;  // output range covered by queued pieces
+  uint32_t src_hi;            // highest source byte + 1 of queued pieces
+  uint64_t s1, s2;            // per-lane Adler-32 partial sums
 };
 
-__device__ __forceinline__ void out_flush(Out &o) {
+__device__ __forceinline__ uint32_t ridx(const Out &o, uint32_t x) {
+  return (x + o.mis) & kRingMask;
+}
+
+__device__ __forceinline__ void out_lits_to_ring(Out &o) {
   if (o.nlit) {
     const uint32_t lane = lane_id();
     const uint64_t p0 = o.pos - o.nlit;
-    if (lane < o.nlit) {
-      const uint32_t b = o.litbuf & 0xff;
-      o.dst[p0 + lane] = (uint8_t)b;
-      o.s1 += b;
-      o.s2 += (p0 + lane) * b;
-    }
+    if (lane < o.nlit) o.ring[ridx(o, p0 + lane)] = (uint8_t)o.litbuf;
     o.nlit = 0;
   }
 }
+
+__device__ __forceinline__ void out_maybe_flush(Out &o);
 
 __device__ __forceinline__ void out_literal(Out &o, uint32_t byte) {
   if (lane_id() == o.nlit) o.litbuf = byte;
   o.nlit++;
   o.pos++;
-  if (o.nlit == (uint32_t)kWave) out_flush(o);
+  if (o.nlit == (uint32_t)kWave) {
+    out_lits_to_ring(o);
+    out_maybe_flush(o);
+  }
 }
 
-__device__ __forceinline__ void out_fence(Out &o) {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  o.synced = o.pos;
-}
-
-// dst[pos, pos+len) = the len bytes starting dist back (periodic when dist < len).
-__device__ __forceinline__ void out_match(Out &o, uint32_t dist, uint32_t len) {
-  out_flush(o);
-  const uint64_t src0 = o.pos - dist;
-  const uint32_t span = dist < len ? dist : len;
-  if (src0 + span > o.synced) out_fence(o);
+// Copies the queued far pieces from HBM into the ring.
+__device__ __forceinline__ void out_resolve(Out &o) {
+  if (!o.np) return;
+  if (o.src_hi > o.retired) {  // their sources must have reached memory
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    o.retired = o.flushed;
+  }
   const uint32_t lane = lane_id();
-  uint8_t *d = o.dst;
-  if (dist >= len) {
-    for (uint32_t i = lane; i < len; i += kWave) {
-      const uint32_t b = d[src0 + i];
-      d[o.pos + i] = (uint8_t)b;
-      o.s1 += b;
-      o.s2 += (o.pos + i) * b;
+  for (uint32_t g = 0; g < o.np; g += kPieceGroup) {
+    uint32_t v[kPieceGroup];
+#pragma unroll
+    for (uint32_t k = 0; k < kPieceGroup; k++) {
+      const uint32_t j = g + k;
+      v[k] = 0;
+      if (j < o.np) {
+        const uint32_t s = __builtin_amdgcn_readlane(o.psrc, j);
+        const uint32_t l = __builtin_amdgcn_readlane(o.plen, j);
+        if (lane < l) v[k] = o.dst[s + lane];
+      }
     }
-  } else {
-    for (uint32_t i = lane; i < len; i += kWave) {
-      const uint32_t b = d[src0 + (i % dist)];
-      d[o.pos + i] = (uint8_t)b;
-      o.s1 += b;
-      o.s2 += (o.pos + i) * b;
+#pragma unroll
+    for (uint32_t k = 0; k < kPieceGroup; k++) {
+      const uint32_t j = g + k;
+      if (j < o.np) {
+        const uint32_t d = __builtin_amdgcn_readlane(o.pdst, j);
+        const uint32_t l = __builtin_amdgcn_readlane(o.plen, j);
+        if (lane < l) o.ring[ridx(o, d + lane)] = (uint8_t)v[k];
+      }
     }
   }
+  o.np = 0;
+  o.hole_lo = 0xffffffffu;
+  o.hole_hi = 0;
+  o.src_hi = 0;
+}
+
+__device__ __forceinline__ void adler_add(Out &o, uint32_t at, uint32_t b) {
+  o.s1 += b;
+  o.s2 += (uint64_t)at * b;
+}
+
+// Stores ring bytes [flushed, end) to HBM; `last` also writes a partial
+// 16-byte tail. Pieces must have been resolved.
+__device__ __forceinline__ void out_store(Out &o, uint32_t end, bool last) {
+  const uint32_t lane = lane_id();
+  uint32_t x = o.flushed;
+  // head bytes up to a 16-byte aligned output address
+  uint32_t head = (16 - ((o.mis + x) & 15)) & 15;
+  if (head > end - x) head = end - x;
+  if (head) {
+    if (lane < head) {
+      const uint32_t b = o.ring[ridx(o, x + lane)];
+      o.dst[x + lane] = (uint8_t)b;
+      adler_add(o, x + lane, b);
+    }
+    x += head;
+  }
+  const uint32_t nvec = (end - x) / 16;
+  for (uint32_t k = lane; k < nvec; k += kWave) {
+    const uint32_t at = x + 16 * k;
+    const uint4 q = *reinterpret_cast<const uint4 *>(o.ring + ridx(o, at));
+    *reinterpret_cast<uint4 *>(o.dst + at) = q;
+    // sum b and sum i*b over the 16 bytes, then s2 += at*sum + sum(i*b)
+    const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+    uint32_t sum = 0, wsum = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const uint32_t b = (w4[i >> 2] >> (8 * (i & 3))) & 0xff;
+      sum += b;
+      wsum += (uint32_t)i * b;
+    }
+    o.s1 += sum;
+    o.s2 += (uint64_t)at * sum + wsum;
+  }
+  x += nvec * 16;
+  if (last && x < end) {
+    const uint32_t tail = end - x;
+    if (lane < tail) {
+      const uint32_t b = o.ring[ridx(o, x + lane)];
+      o.dst[x + lane] = (uint8_t)b;
+      adler_add(o, x + lane, b);
+    }
+    x = end;
+  }
+  o.flushed = x;
+}
+
+__device__ __forceinline__ void out_maybe_flush(Out &o) {
+  if (o.pos - o.flushed >= kFlush) {
+    out_lits_to_ring(o);
+    out_resolve(o);
+    out_store(o, o.pos, false);
+  }
+}
+
+// Appends the len bytes that start dist bytes back (periodic when dist < len).
+__device__ __forceinline__ void out_match(Out &o, uint32_t dist, uint32_t len) {
+  out_lits_to_ring(o);
+  const uint32_t lane = lane_id();
+  const uint32_t src0 = o.pos - dist;
+  if (dist <= kNear) {
+    const uint32_t span = dist < len ? dist : len;
+    if (src0 < o.hole_hi && src0 + span > o.hole_lo) out_resolve(o);
+    __syncthreads();  // ring writes above are visible to the reads below
+    if (dist >= len) {
+      for (uint32_t i = lane; i < len; i += kWave)
+        o.ring[ridx(o, o.pos + i)] = o.ring[ridx(o, src0 + i)];
+    } else {
+      for (uint32_t i = lane; i < len; i += kWave)
+        o.ring[ridx(o, o.pos + i)] = o.ring[ridx(o, src0 + (i % dist))];
+    }
+  } else {  // far: queue <= 64-byte pieces, sources are already in HBM
+    const uint32_t pieces = (len + kWave - 1) / kWave;
+    if (o.np + pieces > (uint32_t)kWave) out_resolve(o);
+    for (uint32_t k = 0; k < pieces; k++) {
+      if (lane == o.np) {
+        o.pdst = o.pos + k * kWave;
+        o.psrc = src0 + k * kWave;
+        o.plen = (len - k * kWave) < (uint32_t)kWave ? (len - k * kWave) : (uint32_t)kWave;
+      }
+      o.np++;
+    }
+    if (o.pos < o.hole_lo) o.hole_lo = o.pos;
+    o.hole_hi = o.pos + len;
+    if (src0 + len > o.src_hi) o.src_hi = src0 + len;
+  }
   o.pos += len;
+  out_maybe_flush(o);
 }
 
 __device__ __forceinline__ void out_reduce_mod(Out &o) {
@@ -219,8 +356,10 @@ __device__ __forceinline__ uint32_t make_entry(uint32_t tab, uint32_t sym, uint3
 // 2^bits entries. Applies zlib's acceptance rules (inftrees.c): an
 // over-subscribed set is rejected; an incomplete set is rejected unless it
 // has a single code of length 1 (literal/distance tables) or no codes at all.
+// Out of line: it runs once per block, and keeping its unrolled code out of
+// the symbol loop leaves that loop's uniform state in scalar registers.
 template <int NSYM>
-__device__ __forceinline__ int32_t build_table(const uint8_t *lens, uint32_t n, uint32_t tab, Huff<NSYM> &h,
+__device__ __noinline__ int32_t build_table(const uint8_t *lens, uint32_t n, uint32_t tab, Huff<NSYM> &h,
                                uint32_t *lut, uint32_t bits) {
   const uint32_t lane = lane_id();
   if (lane < 16) h.count[lane] = 0;
@@ -446,11 +585,18 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
   const uint32_t lane = lane_id();
   Result res{kOk, 0, 1};
   Bits b;
+  // 32-bit bit positions bound the stream to < 2^28 bytes; outputs are
+  // capped below 2^31 bytes.
+  if (len >= (1ull << 28)) {
+    res.status = kCorrupt;
+    return res;
+  }
+  if (cap > 0x7fffffffull) cap = 0x7fffffffull;
   const uintptr_t a = reinterpret_cast<uintptr_t>(src);
   const uint32_t lead = (uint32_t)(a & 3);
   b.w = reinterpret_cast<const uint32_t *>(a - lead);
-  b.limit = ((uint64_t)lead + len) * 8;
-  b.nw = (uint32_t)(((uint64_t)lead + len + 3) / 4);
+  b.limit = (uint32_t)((lead + len) * 8);
+  b.nw = (uint32_t)((lead + len + 3) / 4);
   const bool raw = flags & CODAG_INFLATE_RAW;
   uint64_t start = 0;
   if (!raw) {
@@ -470,11 +616,19 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
 
   Out o;
   o.dst = dst;
-  o.cap = cap;
+  o.ring = S.ring;
+  o.cap = (uint32_t)cap;
   o.pos = 0;
-  o.synced = 0;
+  o.flushed = 0;
+  o.retired = 0;
+  o.mis = (uint32_t)(reinterpret_cast<uintptr_t>(dst) & 15);
   o.litbuf = 0;
   o.nlit = 0;
+  o.np = 0;
+  o.pdst = o.psrc = o.plen = 0;
+  o.hole_lo = 0xffffffffu;
+  o.hole_hi = 0;
+  o.src_hi = 0;
   o.s1 = 0;
   o.s2 = 0;
 
@@ -499,24 +653,23 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
       const uint64_t at = bits_pos(b) / 8 - lead;  // stream byte offset of the data
       if (at + ln > len) { st = kTrunc; break; }
       if (o.pos + ln > o.cap) { st = kOverflow; break; }
-      out_flush(o);
-      for (uint32_t i = lane; i < ln; i += kWave) {
-        const uint32_t v = src[at + i];
-        dst[o.pos + i] = (uint8_t)v;
-        o.s1 += v;
-        o.s2 += (o.pos + i) * v;
+      out_lits_to_ring(o);
+      for (uint32_t done = 0; done < ln;) {  // through the ring, <= 256 bytes per step
+        const uint32_t step = (ln - done) < 256u ? (ln - done) : 256u;
+        for (uint32_t i = lane; i < step; i += kWave)
+          o.ring[ridx(o, o.pos + i)] = src[at + done + i];
+        o.pos += step;
+        done += step;
+        out_maybe_flush(o);
       }
-      o.pos += ln;
       bits_seek(b, lead + at + ln);
     } else if (type == 1) {  // fixed Huffman
       if (tables != 1) {
-        out_flush(o);
         load_fixed(S);
         tables = 1;
       }
       st = decode_block(b, o, S);
     } else if (type == 2) {  // dynamic Huffman
-      out_flush(o);
       uint32_t nlen = 0;
       tables = 0;
       st = read_dynamic(b, S, &nlen);
@@ -524,10 +677,13 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
     } else {
       st = kCorrupt;  // zlib: "invalid block type"
     }
-    out_flush(o);
     out_reduce_mod(o);
   }
-  out_flush(o);
+  // drain: pending literals and far pieces, then the last partial flush
+  out_lits_to_ring(o);
+  out_resolve(o);
+  __syncthreads();
+  out_store(o, o.pos, true);
   // wave reduction of the Adler-32 partial sums
   uint64_t s1 = o.s1 % 65521u, s2 = o.s2 % 65521u;
 #pragma unroll
@@ -558,7 +714,7 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
   return res;
 }
 
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMinWavesPerEu)))
 inflate_chunks_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                       const uint64_t *__restrict__ off, uint64_t n, uint64_t chunk_bytes, uint8_t *__restrict__ out,
                       uint64_t *__restrict__ out_len, int32_t *__restrict__ status, uint32_t flags) {
@@ -577,7 +733,7 @@ inflate_chunks_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   }
 }
 
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMinWavesPerEu)))
 inflate_batch_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
                      codag_stream_result *__restrict__ results, uint32_t flags,
                      uint32_t *__restrict__ counter) {

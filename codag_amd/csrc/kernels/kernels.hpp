@@ -23,6 +23,12 @@ hipError_t launch_rlev2_decode(const uint8_t *comp, uint64_t comp_bytes, const u
                                uint64_t chunk_bytes, uint8_t *out, int32_t *status,
                                uint32_t nt, uint32_t stage, hipStream_t stream);
 
+// RLE v2, one wave64 per chunk; win = LDS window bytes (4096 or 8192).
+hipError_t launch_rlev2_wave(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
+                             const uint64_t *col_len, uint64_t n_chunks, uint32_t w,
+                             uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t win,
+                             hipStream_t stream);
+
 hipError_t launch_inflate_chunks(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *off,
                                  uint64_t n,
                                  uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,

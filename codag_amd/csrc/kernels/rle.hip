@@ -349,9 +349,11 @@ rlev2_decode_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
           const uint32_t nd8 = (len8 - 2) & 0xff;
           uint32_t q = p + 2;
           for (int v = 0; v < 2 && fits; v++) {  // skip uvarint base, svarint delta
+            const uint32_t q0 = q;
             while (q < P && L.sbuf[q] >= 0x80) q++;
             if (q >= P) fits = false;
             q++;
+            if (fits && q - q0 > 10) st = kCorrupt;  // longer than any 64-bit varint
           }
           cnt = 2 + nd8;
           size = (q - p) + (enc ? (nd8 * fbw + 7) / 8 : 0);

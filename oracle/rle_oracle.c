@@ -181,13 +181,17 @@ static uint64_t v2_read_bits(v2_reader *r, v2_bits *b, uint32_t n) {
 /* decoder_new.h:979-991 read_uvarint: 7-bit groups, LSB first, into uint64. */
 static uint64_t v2_uvarint(v2_reader *r) {
   uint64_t v = 0;
-  uint32_t off = 0;
+  uint32_t off = 0, n = 0;
   uint8_t b;
   do {
     b = v2_byte(r);
     v |= shl64((uint64_t)(b & 0x7f), off);
     off += 7;
+    n++;
   } while (b >= 0x80 && !r->err);
+  /* The reference keeps reading continuation bytes on malformed input; this
+   * project rejects varints longer than the 10 bytes a 64-bit value needs. */
+  if (n > 10 && !r->err) r->err = ORACLE_CORRUPT;
   return v;
 }
 

@@ -246,6 +246,8 @@ V2_BAD = [
     ("5e035ca1", 64),          # DIRECT truncated
     ("0a2710", 3),             # overflow
     ("c6", 64),                # DELTA header truncated
+    ("c002" + "80" * 10 + "0102", 64),  # 11-byte varint
+    ("c002" + "80" * 12, 64),  # varint runs off the stream
 ]
 
 
@@ -258,6 +260,18 @@ def test_v2_error_status_matches_oracle(ctx, k):
     st, _ = rle_decode(ctx, ca.RLEV2, comp, blk, cl, 8, cap * 8)
     rc, _ = oracle.rlev2_decode_chunk(s, 8, cap)
     assert rc != 0 and st.tolist() == [rc]
+
+
+@pytest.mark.parametrize("shape", [(64, 4096), (64, 8192), (256, 4096), (128, 4096), (512, 8192)])
+def test_v2_every_launch_shape(ctx, shape):
+    say(f"rlev2 launch shape {shape}")
+    raw, comp, blk, cl, n = ca.synth_rlev2(12 * CH // 8 + 5, 8, CH, seed=77)
+    ctx.tune_set(ca.RLEV2, ca.TuneParams(shape[0], shape[1], 0, 0))
+    try:
+        st, out = _check_chunks(ctx, ca.RLEV2, comp, blk, cl, 8, CH)
+        assert np.all(st == 0) and out[: len(raw)].tobytes() == raw.tobytes()
+    finally:
+        ctx.tune_set(ca.RLEV2, ca.TuneParams(64, 4096, 0, 0))
 
 
 def test_v2_patch_outside_block_is_corrupt(ctx):

@@ -270,3 +270,13 @@ def test_v1_overlong_varint_is_corrupt():
     stream = bytes([0xFF]) + bytes([0x80] * 10) + bytes([0x01])
     rc, _ = oracle.rlev1_decode_chunk(stream, 8, 8)
     assert rc == oracle.CORRUPT
+
+
+def test_v2_overlong_varint_is_corrupt():
+    # DELTA base varint of 11 bytes
+    s = bytes([0xC0, 0x02]) + bytes([0x80] * 10) + bytes([0x01]) + pc.uleb(2)
+    rc, _ = oracle.rlev2_decode_chunk(s, 8, 64)
+    assert rc == oracle.CORRUPT
+    # truncated inside the varint: TRUNCATED wins
+    rc, _ = oracle.rlev2_decode_chunk(bytes([0xC0, 0x02]) + bytes([0x80] * 12), 8, 64)
+    assert rc == oracle.TRUNCATED
