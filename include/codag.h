@@ -116,7 +116,8 @@ codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_
 
 /* Uneven batch of independent zlib streams (no reference counterpart: the
  * reference has fixed-size chunks only; shape follows cudf_deflate.cuh:236-250).
- * One persistent launch pulls streams from a device work queue, largest first. */
+ * One persistent launch pulls streams from a device work queue in descriptor
+ * order (listing the largest streams first gives the best load balance). */
 typedef struct codag_stream_desc {
   const uint8_t *src; /* device */
   uint64_t src_bytes;
