@@ -178,7 +178,11 @@ def bench_rle(ctx, args, cfg):
     gbs = out_bytes / (ms_step / 1e3) / 1e9
     k_ms = per[dom] / args.steps
     achieved = cols[dom].alg_bytes / (k_ms / 1e3) / 1e9
-    kname = "rlev1_decode_kernel" if cols[dom].codec == ca.RLEV1 else "rlev2_decode_kernel"
+    if cols[dom].codec == ca.RLEV1:
+        kname = "rlev1_decode_kernel"
+    else:  # a 64-thread launch shape is the one-wave-per-chunk kernel
+        kname = ("rlev2_wave_kernel" if ctx.tune_get(ca.RLEV2).threads_per_group == 64
+                 else "rlev2_decode_kernel")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(f"{kname}:{cfg}"),
@@ -268,25 +272,280 @@ def bench_deflate(ctx, args):
     }
 
 
+def _parallel(fn, items):
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(host_threads()) as ex:
+        return list(ex.map(fn, items))
+
+
+def bench_batch(ctx, args):
+    """Uneven batch: n zlib streams, sizes log-uniform 4 KiB..4 MiB, blocks mixed
+    fixed/dynamic/stored 20/75/5, decoded by one persistent graph-replayed launch."""
+    import ctypes
+    rng = np.random.default_rng(0xC0DA6 + 4)
+    n = args.streams
+    sizes = np.exp(rng.uniform(np.log(4096), np.log(4 << 20), n)).astype(np.int64)
+    total = int(sizes.sum())
+    log(f"[bench] generating {n} streams, {total >> 20} MiB")
+    payload = ca.synth_deflate_payload(total, CH, seed=0xC0DA6 + 4)
+    poff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    res = _parallel(lambda i: ca.deflate_compress_mixed(payload[poff[i]:poff[i + 1]], 20, 5, 32768,
+                                                        seed=i), range(n))
+    streams = [r[0] for r in res]
+    mix = np.sum([r[1] for r in res], axis=0)
+    order = np.argsort(-sizes, kind="stable")  # largest first: the queue balances the tail
+    slen = np.array([len(streams[i]) for i in order], np.int64)
+    soff = np.concatenate([[0], np.cumsum(slen)]).astype(np.int64)
+    cap = sizes[order]
+    doff = np.concatenate([[0], np.cumsum(cap)]).astype(np.int64)
+    d_src = ctx.upload(np.concatenate([streams[i] for i in order]))
+    del streams, res
+    d_dst = ctx.alloc(int(doff[-1]))
+    descs = (ca.StreamDesc * n)()
+    for k in range(n):
+        descs[k].src, descs[k].src_bytes = d_src.offset(int(soff[k])), int(slen[k])
+        descs[k].dst, descs[k].dst_cap = d_dst.offset(int(doff[k])), int(cap[k])
+    d_descs = ctx.upload(np.frombuffer(descs, np.uint8))
+    rsz = ctypes.sizeof(ca.StreamResult)
+    d_res = ctx.alloc(n * rsz)
+
+    # autotune the persistent launch on a chunk sample of the same generator
+    tr = ca.synth_deflate_payload(512 * CH, CH, seed=0xC0DA6 + 44)
+    tc, tb, _, tn = ca.deflate_compress_chunks(tr, CH, level=6, threads=host_threads())
+    d_tc, d_tb, d_to = ctx.upload(tc), ctx.upload(tb[1:].astype(np.uint64)), ctx.alloc(tn * CH)
+    best, best_ms = ctx.tune_run(ca.DEFLATE, d_tc, len(tc), d_tb, None, tn, 1, CH, d_to)
+    log(f"[bench] autotuner: {best.groups_per_cu} waves per CU ({best_ms:.2f} ms on the sample)")
+
+    ctx.inflate_batch(d_descs, n, d_res)
+    ctx.sync()
+    rr = np.frombuffer(d_res.download(n * rsz).tobytes(), dtype=np.dtype(
+        [("bytes", "<u8"), ("status", "<i4"), ("adler", "<u4")]))
+    out = d_dst.download(int(doff[-1]))
+    for k in range(n):
+        i = order[k]
+        if rr["status"][k] != 0 or int(rr["bytes"][k]) != sizes[i] or \
+                out[doff[k]:doff[k + 1]].tobytes() != payload[poff[i]:poff[i + 1]].tobytes():
+            raise SystemExit(f"batch bit-exact check failed at stream {i}")
+    crc = crc32(out)
+    del out
+    log("[bench] bit-exact check passed")
+    for _ in range(args.warmup):
+        ctx.inflate_batch(d_descs, n, d_res)
+    ctx.sync()
+    ctx.event(0)
+    for _ in range(args.steps):
+        ctx.inflate_batch(d_descs, n, d_res)
+    ctx.event(1)
+    ms = ctx.elapsed(0, 1) / args.steps
+    gbs = total / (ms / 1e3) / 1e9
+
+    # plain per-file launches (one inflate call per stream) on a seeded sample
+    samp = np.sort(np.random.default_rng(5).choice(n, size=min(n, 64), replace=False))
+    pairs = np.zeros(2 * len(samp), np.uint64)
+    pairs[1::2] = slen[samp]
+    d_pairs = ctx.upload(pairs)
+    d_st1 = ctx.alloc(4 * len(samp))
+
+    def per_file():
+        for j, k in enumerate(samp):
+            ctx.inflate_chunks(d_src.offset(int(soff[k])), int(slen[k]), d_pairs.offset(16 * j), 1,
+                               int(cap[k]), d_dst.offset(int(doff[k])), None, d_st1.offset(4 * j))
+
+    per_file()
+    ctx.sync()
+    ctx.event(2)
+    per_file()
+    ctx.event(3)
+    pf_ms = ctx.elapsed(2, 3)
+    pf_gbs = float(cap[samp].sum()) / (pf_ms / 1e3) / 1e9
+    if not np.all(d_st1.download(4 * len(samp), np.int32) == 0):
+        raise SystemExit("per-file launch failed")
+
+    alg = int(slen.sum()) + total + n * (32 + 16)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = host_threads()
+        cs = samp[:64]
+        src_h = np.concatenate([d_src.download(int(soff[-1]))[soff[k]:soff[k] + slen[k]] for k in cs])
+        so = np.concatenate([[0], np.cumsum(slen[cs])[:-1]]).astype(np.uint64)
+        do = np.concatenate([[0], np.cumsum(cap[cs])[:-1]]).astype(np.uint64)
+        args_c = (src_h, so, slen[cs], do, cap[cs], int(cap[cs].sum()), threads)
+        oracle.inflate_batch(*args_c)
+        times = []
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 10 and len(times) < 20:
+            t1 = time.perf_counter()
+            oracle.inflate_batch(*args_c)
+            times.append(time.perf_counter() - t1)
+        med = float(np.median(times))
+        cpu = {"value": round(float(cap[cs].sum()) / med / 1e9, 3), "unit": "GB/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{len(cs)} of the streams ({int(cap[cs].sum()) >> 20} MiB out), system zlib "
+                         f"{oracle.zlib_version()} inflate on {threads} threads, median of {len(times)}"}
+    return {
+        "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic", "bit_exact": True,
+        "config": {"workload": f"batched uneven deflate, {n} zlib streams log-uniform 4 KiB-4 MiB, "
+                               "block mix fixed/dynamic/stored 20/75/5, one persistent "
+                               "graph-replayed launch",
+                   "decompressed_bytes": total, "compressed_bytes": int(slen.sum()),
+                   "blocks_fixed_dynamic_stored": [int(x) for x in mix], "crc32": crc,
+                   "groups_per_cu": int(best.groups_per_cu),
+                   "per_file_launch_gbs": round(pf_gbs, 3),
+                   "per_file_sample": f"{len(samp)} streams, one inflate launch each"},
+        "roofline": {"bound": "hbm", "achieved": round(alg / (ms / 1e3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("inflate_batch_kernel:c4"),
+                     "kernel": "inflate_batch_kernel", "kernel_ms": round(ms, 4),
+                     "alg_bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_served(ctx, args):
+    """Served mix (40% RLE v1 u64, 30% RLE v2 u32, 30% Deflate; sizes log-uniform
+    64 KiB..256 MiB) through the continuous-batching server, device-resident,
+    200 GB admission budget; Poisson arrivals at 10..90% of the measured peak."""
+    import ctypes
+    rng = np.random.default_rng(0xC0DA6 + 5)
+    n_req = args.requests
+    sizes = np.exp(rng.uniform(np.log(64 << 10), np.log(256 << 20), n_req))
+    kinds = rng.choice(3, size=n_req, p=[0.4, 0.3, 0.3])
+    log(f"[bench] generating {n_req} requests")
+    reqs = []
+    for i in range(n_req):
+        nch = max(1, int(round(sizes[i] / CH)))
+        seed = 0xC0DA6 + 500 + i
+        if kinds[i] == 0:
+            raw, cont, blk, cl, n = ca.synth_rlev1(nch * CH // 8, 8, CH, 1 << 16, seed=seed)
+            comp, off, ln, w, codec = cont[8:], blk[:n].copy(), cl[:n].copy(), 8, ca.RLEV1
+        elif kinds[i] == 1:
+            raw, comp, blk, cl, n = ca.synth_rlev2(nch * CH // 4, 4, CH, seed=seed)
+            off, ln, w, codec = blk[:n].copy(), cl.copy(), 4, ca.RLEV2
+        else:
+            raw = ca.synth_deflate_payload(nch * CH, CH, seed=seed)
+            comp, blk, _, n = ca.deflate_compress_chunks(raw, CH, level=6, threads=host_threads())
+            off, ln, w, codec = blk[1:].copy(), np.zeros(1, np.uint64), 1, ca.DEFLATE
+        d = [ctx.upload(np.ascontiguousarray(comp)), ctx.upload(off), ctx.upload(ln), ctx.alloc(n * CH)]
+        q = ca.Request()
+        q.codec, q.flags, q.elem_bytes, q.chunk_bytes, q.n_chunks = codec, ca.REQ_DEVICE_IO, w, CH, n
+        q.comp, q.comp_bytes = d[0].ptr.value, len(comp)
+        q.blk_off, q.col_len, q.out, q.user_tag = d[1].ptr.value, d[2].ptr.value, d[3].ptr.value, i
+        reqs.append(dict(q=q, d=d, raw=raw, bytes=len(raw), codec=int(codec)))
+    mean_bytes = float(np.mean([r["bytes"] for r in reqs]))
+    total_bytes = sum(r["bytes"] for r in reqs)
+
+    def run(inflight, arrivals=None):
+        cfg = ca.ServerConfig(0, 200 * 10**9, inflight)
+        h = ctypes.c_void_p()
+        ca.check(ca.lib().codag_server_create(ctypes.byref(cfg), ctypes.byref(h)), "server_create")
+        buf = (ca.Completion * 64)()
+        done = []
+        try:
+            t0 = time.perf_counter()
+            rid = ctypes.c_uint64()
+            for k, r in enumerate(reqs):
+                if arrivals is not None:
+                    dt = t0 + arrivals[k] - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                ca.check(ca.lib().codag_server_submit(h, ctypes.byref(r["q"]), ctypes.byref(rid)),
+                         "server_submit")
+            while len(done) < len(reqs):
+                got = ctypes.c_uint32()
+                st = ca.lib().codag_server_poll(h, buf, 64, ctypes.byref(got), 120000)
+                ca.check(st, "server_poll")
+                done += [ca.Completion.from_buffer_copy(buf[j]) for j in range(got.value)]
+            wall = time.perf_counter() - t0
+        finally:
+            ca.lib().codag_server_destroy(h)
+        if any(c.status != 0 or c.failed_chunks for c in done):
+            raise SystemExit("served request failed")
+        lat = np.array([c.latency_seconds for c in done]) * 1e3
+        return wall, lat
+
+    run(8)  # warm-up (device buffers, code objects)
+    for r in reqs:  # bit-exact check of every request's output
+        if r["d"][3].download(r["bytes"]).tobytes() != r["raw"].tobytes():
+            raise SystemExit(f"served output mismatch (request {r['q'].user_tag})")
+        r["raw"] = None
+    log("[bench] bit-exact check passed")
+    wall1, _ = run(1)
+    wall8, lat8 = run(8)
+    peak = total_bytes / wall8
+    levels = []
+    for load in (0.1, 0.3, 0.5, 0.7, 0.9):
+        rate = load * peak / mean_bytes  # requests per second
+        arr = np.cumsum(np.random.default_rng(int(load * 100)).exponential(1.0 / rate, n_req))
+        arr -= arr[0]
+        wall, lat = run(8, arr)
+        levels.append({"load": load, "offered_gbs": round(load * peak / 1e9, 2),
+                       "achieved_gbs": round(total_bytes / wall / 1e9, 2),
+                       "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat, 99)), 3)})
+        log(f"[bench] load {load}: {levels[-1]}")
+    return {
+        "metric": METRIC, "value": round(peak / 1e9, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(wall8 * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "mixed",
+        "data": "synthetic", "bit_exact": True,
+        "config": {"workload": f"served mixed-codec decode, {n_req} requests (40% RLE v1 u64, "
+                               "30% RLE v2 u32, 30% deflate), log-uniform 64 KiB-256 MiB, "
+                               "device-resident, 200 GB HBM budget, 8 slots",
+                   "decompressed_bytes": int(total_bytes),
+                   "closed_loop_gbs": {"slots_8": round(peak / 1e9, 2),
+                                       "slots_1_sequential": round(total_bytes / wall1 / 1e9, 2)},
+                   "closed_loop_p50_ms": round(float(np.percentile(lat8, 50)), 3),
+                   "closed_loop_p99_ms": round(float(np.percentile(lat8, 99)), 3),
+                   "poisson": levels},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--streams", type=int, default=10000, help="c4: number of zlib streams")
+    ap.add_argument("--requests", type=int, default=48, help="c5: requests per load level")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
+                    help="first run the plain copy kernel over MIB MiB (traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the decode shards "
-              "by chunk across GPUs without communication, see DESIGN.md", file=sys.stderr)
+              "by chunk across GPUs without communication", file=sys.stderr)
         sys.exit(2)
     ctx = ca.Context(0)
     name, cus = ctx.info()
     log(f"[bench] device {name}, {cus} CUs")
+    if args.copy_ref:
+        # one launch of the plain copy kernel moving a known byte count: under
+        # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it scales the decode kernels'
+        # counters to bytes
+        nb = args.copy_ref << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        a.fill(1)
+        ctx.copy(b, a, nb)
+        ctx.sync()
+        a.free()
+        b.free()
+        log(f"[bench] reference copy of {nb} bytes done")
     if args.config in ("c1", "c2"):
         res = bench_rle(ctx, args, args.config)
-    else:
+    elif args.config == "c3":
         res = bench_deflate(ctx, args)
+    elif args.config == "c4":
+        res = bench_batch(ctx, args)
+    else:
+        res = bench_served(ctx, args)
     res["config"]["device"] = f"{name} ({cus} CUs)"
     print(json.dumps(res), flush=True)
 

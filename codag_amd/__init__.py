@@ -155,6 +155,7 @@ def lib():
         _sig(L, "codag_copy_h2d", st, [_vp, _vp, _vp, _u64])
         _sig(L, "codag_copy_d2h", st, [_vp, _vp, _vp, _u64])
         _sig(L, "codag_memset_d", st, [_vp, _vp, ctypes.c_int, _u64])
+        _sig(L, "codag_copy_d2d", st, [_vp, _vp, _vp, _u64])
         _sig(L, "codag_sync", st, [_vp])
         _sig(L, "codag_timer_start", st, [_vp])
         _sig(L, "codag_timer_stop", st, [_vp, ctypes.POINTER(ctypes.c_float)])
@@ -234,6 +235,10 @@ class Context:
 
     def sync(self):
         check(lib().codag_sync(self.h), "sync")
+
+    def copy(self, dst, src, nbytes: int):
+        """Asynchronous device-to-device copy by the library's plain copy kernel."""
+        check(lib().codag_copy_d2d(self.h, _dp(dst), _dp(src), nbytes), "copy_d2d")
 
     def timer_start(self):
         check(lib().codag_timer_start(self.h), "timer_start")

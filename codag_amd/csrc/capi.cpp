@@ -79,7 +79,8 @@ codag_status check_rle_args(uint32_t w, uint64_t chunk_bytes) {
 void default_tune(codag_ctx *c) {
   c->tune[CODAG_RLEV1] = {256, 4096, 0, 0};
   c->tune[CODAG_RLEV2] = {64, 4096, 0, 0};
-  c->tune[CODAG_DEFLATE] = {64, 0, 16, 1};
+  // 28 waves per CU: the inflate kernel's LDS and register budget allow seven per SIMD
+  c->tune[CODAG_DEFLATE] = {64, 0, 28, 1};
   c->tune[0] = {0, 0, 0, 0};
 }
 
@@ -461,7 +462,7 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
   DeviceGuard g(ctx->device);
   std::vector<codag_tune_params> cands;
   if (codec == CODAG_DEFLATE) {
-    for (uint32_t per : {4u, 8u, 12u, 16u, 20u}) cands.push_back({64, 0, per, 1});
+    for (uint32_t per : {8u, 16u, 20u, 24u, 28u}) cands.push_back({64, 0, per, 1});
   } else {
     if (codec == CODAG_RLEV2) {  // one-wave-per-chunk shapes first
       cands.push_back({64, 4096, 0, 0});
@@ -475,8 +476,24 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
   std::vector<uint8_t> ref, cur;
   float bestt = 1e30f;
   codag_tune_params bestp = saved;
-  DevBuf d_st, d_off2;
+  DevBuf d_st, d_descs, d_res;
   HIP_TRY(d_st.alloc(n_chunks * 8 + 8));
+  if (codec == CODAG_DEFLATE) {
+    // Deflate candidates differ in the persistent batch launch (waves per CU
+    // pulling from the work queue): time that launch on the sample's chunks.
+    std::vector<uint64_t> off(n_chunks + 1);
+    HIP_TRY(hipMemcpy(off.data(), d_blk_off, 8 * (n_chunks + 1), hipMemcpyDeviceToHost));
+    std::vector<codag_stream_desc> descs(n_chunks);
+    for (uint64_t i = 0; i < n_chunks; i++) {
+      if (off[i + 1] < off[i] || off[i + 1] > comp_bytes)
+        return fail(CODAG_ERR_INVALID, "tune_run: offsets out of range");
+      descs[i] = {d_comp + off[i], off[i + 1] - off[i], d_out + i * chunk_bytes, chunk_bytes};
+    }
+    HIP_TRY(d_descs.alloc(sizeof(codag_stream_desc) * n_chunks + 8));
+    HIP_TRY(d_res.alloc(sizeof(codag_stream_result) * n_chunks + 8));
+    HIP_TRY(hipMemcpy(d_descs.p, descs.data(), sizeof(codag_stream_desc) * n_chunks,
+                      hipMemcpyHostToDevice));
+  }
   for (size_t ci = 0; ci < cands.size(); ci++) {
     ctx->tune[codec] = cands[ci];
     auto run = [&]() -> codag_status {
@@ -486,8 +503,8 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
       if (codec == CODAG_RLEV2)
         return codag_rlev2_decode(ctx, d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w,
                                   chunk_bytes, d_out, d_st.as<int32_t>(), ctx->stream);
-      return codag_inflate_chunks(ctx, d_comp, comp_bytes, d_blk_off, n_chunks, chunk_bytes,
-                                  d_out, nullptr, d_st.as<int32_t>(), 0, ctx->stream);
+      return codag_inflate_batch(ctx, d_descs.as<codag_stream_desc>(), n_chunks,
+                                 d_res.as<codag_stream_result>(), 0, ctx->stream);
     };
     codag_status s = run();  // warm-up
     if (s) {
@@ -552,6 +569,14 @@ codag_status codag_copy_d2h(codag_ctx *ctx, void *h_dst, const void *d_src, uint
   DeviceGuard g(ctx->device);
   HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CODAG_OK;
+}
+
+codag_status codag_copy_d2d(codag_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes) {
+  if (!ctx || (bytes && (!d_dst || !d_src))) return fail(CODAG_ERR_INVALID, "copy_d2d");
+  DeviceGuard g(ctx->device);
+  HIP_TRY(launch_copy(static_cast<uint8_t *>(d_dst), static_cast<const uint8_t *>(d_src), bytes,
+                      ctx->stream));
   return CODAG_OK;
 }
 

@@ -5,19 +5,26 @@
 //    is held in two VGPR windows -- lane i holds dword i of a 256-byte window
 //    -- and refilled into the bit buffer with v_readlane, so a refill costs no
 //    memory round trip; the next window is prefetched with one coalesced load.
-//  * Huffman decode: per-wave LDS lookup tables built per block by all 64
+//  * Huffman tables: per-wave LDS lookup tables built per block by all 64
 //    lanes (10-bit literal/length table, 8-bit distance table, canonical
 //    count/symbol arrays for the rare longer codes). Table entries carry the
-//    RFC 1951 length/distance base and extra-bit count directly.
-//  * Literals: accumulated one byte per lane (lane k holds the k-th) and written
-//    with one coalesced 64-lane store per 64 literals (or before a match).
-//  * LZ77 matches: lanes copy 64 bytes per step from the stream's own output
-//    in global memory (the window is the chunk itself); a workgroup fence
-//    (s_waitcnt) is issued only when a match reads bytes whose stores may not
-//    have retired yet.
-//  * Adler-32: each lane accumulates sum(b) and sum(pos*b) over the bytes it
-//    writes; the wave reduces them at the end, so the trailer is verified
-//    without a second pass over the output.
+//    RFC 1951 length/distance base and extra-bit count directly; a literal's
+//    entry has bit 31 set so the symbol loop tests it with one scalar compare.
+//  * Two phases per batch of up to 64 symbols:
+//      1. the symbol loop (scalar unit) only decodes: one LDS table read per
+//         literal/length and per distance code, and the decoded token is
+//         parked in lane j of a VGPR (token j);
+//      2. the whole wave materialises the batch: a wave prefix sum gives each
+//         token its output offset, literals go to an LDS ring in one store,
+//         matches whose source has left the ring are gathered from HBM (eight
+//         per memory round trip), and the remaining matches are copied
+//         ring -> ring in token order, 64 bytes per step.
+//    Decoding never depends on output bytes, so the scalar loop runs without
+//    waiting on memory beyond its own table reads.
+//  * The ring is written to HBM with 16-byte stores after each batch that
+//    leaves at least kFlushMin unwritten bytes; Adler-32 partial sums (sum b,
+//    sum pos*b) are accumulated on the way, so the trailer is verified without
+//    a second pass over the output.
 //
 // Reference counterpart: include/zlib/zlib.h:807-866 (full_warp_shared_lut),
 // 531-707 (dynamic header / table build), 712-805 (symbol loop), 188-347
@@ -35,21 +42,37 @@ using namespace codag::dev;
 constexpr uint32_t kLitBits = 10;
 constexpr uint32_t kDistBits = 8;
 constexpr uint32_t kCodeBits = 7;
+constexpr uint32_t kLitMask = (1u << kLitBits) - 1;
+constexpr uint32_t kDistMask = (1u << kDistBits) - 1;
 
 enum : int32_t { kOk = 0, kTrunc = 1, kOverflow = 2, kCorrupt = 3 };
+constexpr int32_t kEob = 16;  // symbol loop stop: end of block (not a status)
 enum : uint32_t { kKindLit = 0, kKindLen = 1, kKindEob = 2, kKindBad = 3 };
 enum : uint32_t { kTabLit = 0, kTabDist = 1, kTabCodes = 2 };
 
-// RFC 1951 section 3.2.5 base/extra tables (also include/zlib/zlib.h:35-51).
-__constant__ uint16_t kLenBase[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
-                                      31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
-__constant__ uint8_t kLenExtra[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2,
-                                      2, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
-__constant__ uint16_t kDistBase[30] = {1,    2,    3,    4,    5,    7,     9,     13,    17,  25,
-                                       33,   49,   65,   97,   129,  193,   257,   385,   513, 769,
-                                       1025, 1537, 2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577};
-__constant__ uint8_t kDistExtra[30] = {0, 0, 0, 0, 1, 1, 2, 2,  3,  3,  4,  4,  5,  5,  6,
-                                       6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+// 16-bit LUT entries (bits 3:0 always hold the code length; 0 = not in LUT):
+//   literal      : bit 15 set, byte in bits 11:4
+//   length code c: extra-bit count in bits 6:4, c = symbol-257 in bits 11:7,
+//                  kind (kKindLen / kKindEob / kKindBad) in bits 13:12
+//   distance d   : extra-bit count in bits 7:4, d in bits 12:8, bit 13 = invalid (d >= 30)
+//   code-length  : symbol in bits 12:8
+constexpr uint32_t kLitFlag = 0x8000u;
+constexpr uint32_t kDistBad = 0x2000u;
+
+// RFC 1951 section 3.2.5 length and distance codes (the tables of
+// include/zlib/zlib.h:35-51) in closed form: length code c (0..28) has
+// (c - 4) / 4 extra bits from c = 8 on (none for c = 28, length 258) and base
+// 3 + c below 8, else 3 + ((4 + c % 4) << extra); distance code d (0..29) has
+// (d - 2) / 2 extra bits from d = 4 on and base 1 + d below 4, else
+// 1 + ((2 + d % 2) << extra).
+__device__ __forceinline__ uint32_t len_extra(uint32_t c) { return (c < 8 || c == 28) ? 0 : (c - 4) >> 2; }
+__device__ __forceinline__ uint32_t len_base(uint32_t c, uint32_t extra) {
+  return c < 8 ? c + 3 : (c == 28 ? 258 : ((4 + (c & 3)) << extra) + 3);
+}
+__device__ __forceinline__ uint32_t dist_extra(uint32_t d) { return d < 4 ? 0 : (d - 2) >> 1; }
+__device__ __forceinline__ uint32_t dist_base(uint32_t d, uint32_t extra) {
+  return d < 4 ? d + 1 : ((2 + (d & 1)) << extra) + 1;
+}
 // Order of the code-length code lengths (RFC 1951 3.2.7).
 __constant__ uint8_t kClOrder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
 
@@ -60,18 +83,40 @@ template <int NSYM> struct Huff {
   uint16_t sym[NSYM];
 };
 
-constexpr uint32_t kRingBytes = 4096;  // output ring (see the output side below)
+// Output ring and batch limits. A batch holds at most 64 tokens and stops once
+// its matches reach kBatchBytes, so it produces T < kBatchBytes + 258 + 64
+// bytes. At a batch start fewer than kFlushMin bytes are unwritten to HBM.
+// Then (a) the batch never overwrites a ring byte that is not yet in HBM and
+// (b) a match whose source lies before the ring's oldest intact byte reads
+// only bytes that are already in HBM, when kRing >= T + 258 + kFlushMin.
+// Sized so a wave needs under 6 KiB of LDS (about seven waves per SIMD).
+constexpr uint32_t kRing = 2048;
+constexpr uint32_t kRingMask = kRing - 1;
+constexpr uint32_t kBatchBytes = 512;
+constexpr uint32_t kFlushMin = 256;
+static_assert(kRing >= kBatchBytes + 258 + 64 + 258 + kFlushMin, "ring too small for the batch");
+// A symbol reads at most 15 + 5 + 15 + 13 bits: a batch that starts this far
+// from the stream end cannot read past it.
+constexpr uint32_t kCarefulBits = 64 * 48;
+constexpr uint32_t kGather = 8;  // far-match pieces in flight per memory round trip
+// The serial symbol loop is latency-bound, so throughput grows with the waves
+// resident per SIMD: LDS (under 6 KiB per wave) allows about seven, and the
+// register budget is held to match.
+constexpr int kWavesPerSimd = 7;
 
 struct Lds {
-  alignas(16) uint8_t ring[kRingBytes];
-  uint32_t lit[1u << kLitBits];
-  uint32_t dist[1u << kDistBits];
+  alignas(16) uint8_t ring[kRing];
+  uint16_t lit[1u << kLitBits];
+  uint16_t dist[1u << kDistBits];
   Huff<288> hl;
   Huff<32> hd;
   uint8_t lens[320];
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t j) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
+}
 
 // ------------------------------------------------------------- bit reader
 
@@ -82,29 +127,37 @@ struct Bits {
   uint32_t next;      // next dword to append to buf
   uint64_t buf;
   uint32_t cnt;
-  uint32_t wbase;     // dword index held by lane 0 of win
-  uint32_t win, win2; // dwords [wbase, wbase+64) and [wbase+64, wbase+128)
+  uint32_t wbase;     // first dword of the current 64-dword block (a multiple of 64)
+  // Blocks of 64 dwords, lane i holding dword i: the block at dword 64*m lives
+  // in w0 for even m and in w1 for odd m. The current block and the next one
+  // are resident; a register is reloaded only when its block is used up, so
+  // the windows never move between registers.
+  uint32_t w0, w1;
 };
 
+__device__ __forceinline__ uint32_t bits_block(const Bits &b, uint32_t first) {
+  const uint32_t i = first + lane_id();
+  return i < b.nw ? b.w[i] : 0u;
+}
+
 __device__ __forceinline__ void bits_windows(Bits &b) {
-  const uint32_t lane = lane_id();
-  uint32_t i = b.wbase + lane;
-  b.win = i < b.nw ? b.w[i] : 0u;
-  i += kWave;
-  b.win2 = i < b.nw ? b.w[i] : 0u;
+  const uint32_t cur = bits_block(b, b.wbase), nxt = bits_block(b, b.wbase + kWave);
+  const bool odd = (b.wbase / kWave) & 1;
+  b.w0 = odd ? nxt : cur;
+  b.w1 = odd ? cur : nxt;
 }
 
 __device__ __forceinline__ void bits_refill(Bits &b) {
   if (b.cnt <= 32) {
     uint32_t k = b.next - b.wbase;
     if (k >= (uint32_t)kWave) {
-      b.win = b.win2;
+      // the used-up block's register takes the block after the next one
+      if ((b.wbase / kWave) & 1) b.w1 = bits_block(b, b.wbase + 2 * kWave);
+      else b.w0 = bits_block(b, b.wbase + 2 * kWave);
       b.wbase += kWave;
-      const uint32_t i = b.wbase + kWave + lane_id();
-      b.win2 = i < b.nw ? b.w[i] : 0u;
       k -= kWave;
     }
-    const uint32_t d = __builtin_amdgcn_readlane(b.win, k);
+    const uint32_t d = ((b.wbase / kWave) & 1) ? rdlane(b.w1, k) : rdlane(b.w0, k);
     b.buf |= (uint64_t)d << b.cnt;
     b.cnt += 32;
     b.next++;
@@ -138,114 +191,33 @@ __device__ __forceinline__ void bits_seek(Bits &b, uint64_t lead_plus_at) {
 // The most recent kRing output bytes live in an LDS ring. Byte x of the
 // stream sits at ring[(x + mis) & (kRing-1)], where mis = dst & 15, so a
 // 16-byte aligned output address is also 16-byte aligned in the ring.
-//  * literals are buffered one per lane and written to the ring in one
-//    64-lane LDS store;
-//  * matches with distance <= kNear copy ring -> ring (LDS only);
-//  * farther matches read bytes that were flushed to HBM long ago; they are
-//    queued as pieces of <= 64 bytes (lane j holds piece j) and resolved in
-//    groups of 16 pieces per memory round trip;
-//  * every kFlush bytes the ring is written to HBM with 16-byte stores, and
-//    Adler-32 partial sums (sum b, sum pos*b) are accumulated on the way.
-// A global load waits for all earlier stores of the wave (one in-order
-// counter), so the design keeps global reads rare and batched.
-constexpr uint32_t kRing = kRingBytes;
-constexpr uint32_t kRingMask = kRing - 1;
-constexpr uint32_t kFlush = 1024;
-constexpr uint32_t kNear = kRing - 258 - 64 - kFlush;  // ring copies never read overwritten bytes
-constexpr uint32_t kPieceGroup = 8;
-// Latency of the serial symbol loop is hidden by other waves on the SIMD:
-// keep register use low enough for this many waves per SIMD.
-constexpr int kMinWavesPerEu = 6;
-
-// Positions are 32-bit (streams and outputs below 2^31 bytes) so the
-// wave-uniform state fits in scalar registers without spilling.
+// Positions are 32-bit (outputs below 2^31 bytes) so the wave-uniform state
+// stays in scalar registers.
 struct Out {
   uint8_t *dst;      // global output
   uint8_t *ring;     // LDS ring
   uint32_t cap;
-  uint32_t pos;      // bytes produced (including buffered literals)
+  uint32_t pos;      // bytes produced
   uint32_t flushed;  // bytes [0, flushed) have been stored to dst
-  uint32_t retired;  // bytes [0, retired) are known to have reached memory
   uint32_t mis;      // dst & 15
-  uint32_t litbuf;   // pending literal byte of this lane
-  uint32_t nlit;     // pending literals (uniform)
-  uint32_t np;       // queued far pieces (uniform)
-  uint32_t pdst, psrc, plen;  // piece `lane` (valid for lane < np)
-  uint32_t hole_lo, hole_hi;  // output range covered by queued pieces
-  uint32_t src_hi;            // highest source byte + 1 of queued pieces
-  uint64_t s1, s2;            // per-lane Adler-32 partial sums
+  // Per-lane Adler-32 partial sums: s1 = sum b, s2 = sum pos*b (mod 65521),
+  // both reduced after every store. One store adds at most ~8e8 to s2 (at
+  // most 15 single bytes and two 16-byte groups per lane), so 32 bits hold.
+  uint32_t s1, s2;
 };
 
 __device__ __forceinline__ uint32_t ridx(const Out &o, uint32_t x) {
   return (x + o.mis) & kRingMask;
 }
 
-__device__ __forceinline__ void out_lits_to_ring(Out &o) {
-  if (o.nlit) {
-    const uint32_t lane = lane_id();
-    const uint64_t p0 = o.pos - o.nlit;
-    if (lane < o.nlit) o.ring[ridx(o, p0 + lane)] = (uint8_t)o.litbuf;
-    o.nlit = 0;
-  }
-}
-
-__device__ __forceinline__ void out_maybe_flush(Out &o);
-
-__device__ __forceinline__ void out_literal(Out &o, uint32_t byte) {
-  if (lane_id() == o.nlit) o.litbuf = byte;
-  o.nlit++;
-  o.pos++;
-  if (o.nlit == (uint32_t)kWave) {
-    out_lits_to_ring(o);
-    out_maybe_flush(o);
-  }
-}
-
-// Copies the queued far pieces from HBM into the ring.
-__device__ __forceinline__ void out_resolve(Out &o) {
-  if (!o.np) return;
-  if (o.src_hi > o.retired) {  // their sources must have reached memory
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    o.retired = o.flushed;
-  }
-  const uint32_t lane = lane_id();
-  for (uint32_t g = 0; g < o.np; g += kPieceGroup) {
-    uint32_t v[kPieceGroup];
-#pragma unroll
-    for (uint32_t k = 0; k < kPieceGroup; k++) {
-      const uint32_t j = g + k;
-      v[k] = 0;
-      if (j < o.np) {
-        const uint32_t s = __builtin_amdgcn_readlane(o.psrc, j);
-        const uint32_t l = __builtin_amdgcn_readlane(o.plen, j);
-        if (lane < l) v[k] = o.dst[s + lane];
-      }
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kPieceGroup; k++) {
-      const uint32_t j = g + k;
-      if (j < o.np) {
-        const uint32_t d = __builtin_amdgcn_readlane(o.pdst, j);
-        const uint32_t l = __builtin_amdgcn_readlane(o.plen, j);
-        if (lane < l) o.ring[ridx(o, d + lane)] = (uint8_t)v[k];
-      }
-    }
-  }
-  o.np = 0;
-  o.hole_lo = 0xffffffffu;
-  o.hole_hi = 0;
-  o.src_hi = 0;
-}
-
 __device__ __forceinline__ void adler_add(Out &o, uint32_t at, uint32_t b) {
   o.s1 += b;
-  o.s2 += (uint64_t)at * b;
+  o.s2 += (at % 65521u) * b;
 }
 
 // Stores ring bytes [flushed, end) to HBM; `last` also writes a partial
-// 16-byte tail. Pieces must have been resolved.
-__device__ __forceinline__ void out_store(Out &o, uint32_t end, bool last) {
-  const uint32_t lane = lane_id();
+// 16-byte tail.
+__device__ __forceinline__ void out_store(Out &o, uint32_t lane, uint32_t end, bool last) {
   uint32_t x = o.flushed;
   // head bytes up to a 16-byte aligned output address
   uint32_t head = (16 - ((o.mis + x) & 15)) & 15;
@@ -263,17 +235,26 @@ __device__ __forceinline__ void out_store(Out &o, uint32_t end, bool last) {
     const uint32_t at = x + 16 * k;
     const uint4 q = *reinterpret_cast<const uint4 *>(o.ring + ridx(o, at));
     *reinterpret_cast<uint4 *>(o.dst + at) = q;
-    // sum b and sum i*b over the 16 bytes, then s2 += at*sum + sum(i*b)
-    const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
-    uint32_t sum = 0, wsum = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      const uint32_t b = (w4[i >> 2] >> (8 * (i & 3))) & 0xff;
-      sum += b;
-      wsum += (uint32_t)i * b;
-    }
+    // sum b and sum i*b over the 16 bytes (v_dot4_u32_u8 against the byte
+    // weights), then s2 += at*sum + sum(i*b)
+    const uint32_t sum = __builtin_amdgcn_udot4(
+        q.x, 0x01010101u,
+        __builtin_amdgcn_udot4(q.y, 0x01010101u,
+                               __builtin_amdgcn_udot4(q.z, 0x01010101u,
+                                                      __builtin_amdgcn_udot4(q.w, 0x01010101u, 0u, false),
+                                                      false),
+                               false),
+        false);
+    const uint32_t wsum = __builtin_amdgcn_udot4(
+        q.x, 0x03020100u,
+        __builtin_amdgcn_udot4(q.y, 0x07060504u,
+                               __builtin_amdgcn_udot4(q.z, 0x0b0a0908u,
+                                                      __builtin_amdgcn_udot4(q.w, 0x0f0e0d0cu, 0u, false),
+                                                      false),
+                               false),
+        false);
     o.s1 += sum;
-    o.s2 += (uint64_t)at * sum + wsum;
+    o.s2 += (at % 65521u) * sum + wsum;
   }
   x += nvec * 16;
   if (last && x < end) {
@@ -286,54 +267,103 @@ __device__ __forceinline__ void out_store(Out &o, uint32_t end, bool last) {
     x = end;
   }
   o.flushed = x;
-}
-
-__device__ __forceinline__ void out_maybe_flush(Out &o) {
-  if (o.pos - o.flushed >= kFlush) {
-    out_lits_to_ring(o);
-    out_resolve(o);
-    out_store(o, o.pos, false);
-  }
-}
-
-// Appends the len bytes that start dist bytes back (periodic when dist < len).
-__device__ __forceinline__ void out_match(Out &o, uint32_t dist, uint32_t len) {
-  out_lits_to_ring(o);
-  const uint32_t lane = lane_id();
-  const uint32_t src0 = o.pos - dist;
-  if (dist <= kNear) {
-    const uint32_t span = dist < len ? dist : len;
-    if (src0 < o.hole_hi && src0 + span > o.hole_lo) out_resolve(o);
-    __syncthreads();  // ring writes above are visible to the reads below
-    if (dist >= len) {
-      for (uint32_t i = lane; i < len; i += kWave)
-        o.ring[ridx(o, o.pos + i)] = o.ring[ridx(o, src0 + i)];
-    } else {
-      for (uint32_t i = lane; i < len; i += kWave)
-        o.ring[ridx(o, o.pos + i)] = o.ring[ridx(o, src0 + (i % dist))];
-    }
-  } else {  // far: queue <= 64-byte pieces, sources are already in HBM
-    const uint32_t pieces = (len + kWave - 1) / kWave;
-    if (o.np + pieces > (uint32_t)kWave) out_resolve(o);
-    for (uint32_t k = 0; k < pieces; k++) {
-      if (lane == o.np) {
-        o.pdst = o.pos + k * kWave;
-        o.psrc = src0 + k * kWave;
-        o.plen = (len - k * kWave) < (uint32_t)kWave ? (len - k * kWave) : (uint32_t)kWave;
-      }
-      o.np++;
-    }
-    if (o.pos < o.hole_lo) o.hole_lo = o.pos;
-    o.hole_hi = o.pos + len;
-    if (src0 + len > o.src_hi) o.src_hi = src0 + len;
-  }
-  o.pos += len;
-  out_maybe_flush(o);
-}
-
-__device__ __forceinline__ void out_reduce_mod(Out &o) {
   o.s1 %= 65521u;
   o.s2 %= 65521u;
+}
+
+// Copies far matches (lanes in fm) from HBM into the ring. The first 64 bytes
+// of kGather matches are loaded before any is written, so a group costs one
+// memory round trip; the rest of the (rare) longer far matches follows.
+__device__ __forceinline__ void gather_far(Out &o, uint32_t lane, uint64_t fm, uint32_t x,
+                                           uint32_t dist, uint32_t mlen) {
+  uint64_t g = fm;
+  while (g) {
+    uint32_t v[kGather];
+    uint64_t h = g;
+#pragma unroll
+    for (uint32_t k = 0; k < kGather; k++) {
+      v[k] = 0;
+      if (h) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(h);
+        h &= h - 1;
+        const uint32_t sj = rdlane(x, j) - rdlane(dist, j), lj = rdlane(mlen, j);
+        if (lane < lj) v[k] = o.dst[sj + lane];
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kGather; k++) {
+      if (g) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(g);
+        g &= g - 1;
+        const uint32_t xj = rdlane(x, j), lj = rdlane(mlen, j);
+        if (lane < lj) o.ring[ridx(o, xj + lane)] = (uint8_t)v[k];
+      }
+    }
+  }
+  uint64_t lm = fm & __ballot(mlen > (uint32_t)kWave);
+  while (lm) {
+    const uint32_t j = (uint32_t)__builtin_ctzll(lm);
+    lm &= lm - 1;
+    const uint32_t xj = rdlane(x, j), dj = rdlane(dist, j), lj = rdlane(mlen, j);
+#pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
+    for (uint32_t i = kWave + lane; i < lj; i += kWave) o.ring[ridx(o, xj + i)] = o.dst[xj - dj + i];
+  }
+}
+
+// Materialises a batch of nt tokens (lane j holds token j):
+//   literal: bit 31 set, byte in bits 27:20 (its LUT entry << 16);
+//   match: length in bits 8:0, distance in bits 24:9.
+// Errors are reported for the first offending token, in stream order:
+// output past cap (overflow) or a distance beyond the bytes produced so far.
+__device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t tok, uint32_t nt) {
+  if (nt == 0) return kOk;
+  const bool act = lane < nt;
+  const bool lit = act && (int32_t)tok < 0;
+  const bool mat = act && (int32_t)tok >= 0;
+  const uint32_t mlen = tok & 0x1ff;
+  const uint32_t dist = (tok >> 9) & 0xffff;
+  const uint32_t tl = lit ? 1u : (mat ? mlen : 0u);
+  const uint32_t incl = wave_incl_scan(tl);
+  const uint32_t x = o.pos + incl - tl;  // output position of this token
+  uint32_t err = 0;
+  if (act && x + tl > o.cap) err = kOverflow;
+  else if (mat && dist > x) err = kCorrupt;  // zlib: "invalid distance too far back"
+  const uint64_t em = __ballot(err != 0);
+  if (em) return (int32_t)rdlane(err, (uint32_t)__builtin_ctzll(em));
+  const uint32_t pend = o.pos + rdlane(incl, nt - 1);
+
+  if (lit) o.ring[ridx(o, x)] = (uint8_t)(tok >> 20);
+  // a source byte y is intact in the ring iff y >= pend - kRing
+  const bool far = mat && (x - dist) + kRing < pend;
+  const uint64_t fm = __ballot(far);
+  if (fm) gather_far(o, lane, fm, x, dist, mlen);
+  uint64_t nm = __ballot(mat && !far);
+  while (nm) {  // in token order: a match may read bytes of earlier tokens
+    const uint32_t j = (uint32_t)__builtin_ctzll(nm);
+    nm &= nm - 1;
+    const uint32_t xj = rdlane(x, j), dj = rdlane(dist, j), lj = rdlane(mlen, j);
+    const uint32_t sj = xj - dj;
+    if (dj >= (uint32_t)kWave) {
+      // byte i >= dj repeats byte i - dj, written one 64-byte step earlier
+#pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
+      for (uint32_t i = lane; i < lj; i += kWave) o.ring[ridx(o, xj + i)] = o.ring[ridx(o, sj + i)];
+    } else {
+      // periodic copy: byte i = source byte i mod dj. The float quotient is
+      // exact up to one unit for i < 2^9, dj < 64; the remainder is corrected.
+      const float rd = __builtin_amdgcn_rcpf((float)dj);
+#pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
+      for (uint32_t i = lane; i < lj; i += kWave) {
+        const uint32_t q = (uint32_t)((float)i * rd);
+        int32_t m = (int32_t)(i - q * dj);
+        if (m < 0) m += (int32_t)dj;
+        if (m >= (int32_t)dj) m -= (int32_t)dj;
+        o.ring[ridx(o, xj + i)] = o.ring[ridx(o, sj + (uint32_t)m)];
+      }
+    }
+  }
+  o.pos = pend;
+  if (pend - o.flushed >= kFlushMin) out_store(o, lane, pend, false);
+  return kOk;
 }
 
 // ------------------------------------------------------------- table build
@@ -341,15 +371,16 @@ __device__ __forceinline__ void out_reduce_mod(Out &o) {
 __device__ __forceinline__ uint32_t make_entry(uint32_t tab, uint32_t sym, uint32_t len) {
   if (tab == kTabCodes) return len | (sym << 8);
   if (tab == kTabDist) {
-    if (sym >= 30) return len | (kKindBad << 4);
-    return len | ((uint32_t)kDistBase[sym] << 8) | ((uint32_t)kDistExtra[sym] << 24);
+    if (sym >= 30) return len | kDistBad;
+    return len | (dist_extra(sym) << 4) | (sym << 8);
   }
-  if (sym < 256) return len | (kKindLit << 4) | (sym << 8);
-  if (sym == 256) return len | (kKindEob << 4);
-  if (sym <= 285)
-    return len | (kKindLen << 4) | ((uint32_t)kLenBase[sym - 257] << 8) |
-           ((uint32_t)kLenExtra[sym - 257] << 20);
-  return len | (kKindBad << 4);
+  if (sym < 256) return len | (sym << 4) | kLitFlag;
+  if (sym == 256) return len | (kKindEob << 12);
+  if (sym <= 285) {
+    const uint32_t c = sym - 257;
+    return len | (len_extra(c) << 4) | (c << 7) | (kKindLen << 12);
+  }
+  return len | (kKindBad << 12);
 }
 
 // Builds the canonical tables for n code lengths in lens[] and the LUT of
@@ -360,7 +391,7 @@ __device__ __forceinline__ uint32_t make_entry(uint32_t tab, uint32_t sym, uint3
 // the symbol loop leaves that loop's uniform state in scalar registers.
 template <int NSYM>
 __device__ __noinline__ int32_t build_table(const uint8_t *lens, uint32_t n, uint32_t tab, Huff<NSYM> &h,
-                               uint32_t *lut, uint32_t bits) {
+                               uint16_t *lut, uint32_t bits) {
   const uint32_t lane = lane_id();
   if (lane < 16) h.count[lane] = 0;
   for (uint32_t i = lane; i < (1u << bits); i += kWave) lut[i] = 0;
@@ -477,7 +508,7 @@ __device__ __forceinline__ int32_t read_dynamic(Bits &b, Lds &S, uint32_t *nlen_
     S.lens[kClOrder[lane]] = (uint8_t)v;
   }
   __syncthreads();
-  int32_t st = build_table(S.lens, 19, kTabCodes, S.hl, S.lit, kCodeBits);
+  int32_t st = (int32_t)uni((uint32_t)build_table(S.lens, 19, kTabCodes, S.hl, S.lit, kCodeBits));
   if (st != kOk) return st;
   // literal/length + distance code lengths with repeat codes 16/17/18
   const uint32_t total = nlen + ndist;
@@ -517,9 +548,9 @@ __device__ __forceinline__ int32_t read_dynamic(Bits &b, Lds &S, uint32_t *nlen_
   if (bits_past_end(b)) return kTrunc;
   __syncthreads();
   if (uni(S.lens[256]) == 0) return kCorrupt;  // zlib: "missing end-of-block"
-  st = build_table(S.lens, nlen, kTabLit, S.hl, S.lit, kLitBits);
+  st = (int32_t)uni((uint32_t)build_table(S.lens, nlen, kTabLit, S.hl, S.lit, kLitBits));
   if (st != kOk) return st;
-  st = build_table(S.lens + nlen, ndist, kTabDist, S.hd, S.dist, kDistBits);
+  st = (int32_t)uni((uint32_t)build_table(S.lens + nlen, ndist, kTabDist, S.hd, S.dist, kDistBits));
   *nlen_out = nlen;
   return st;
 }
@@ -535,48 +566,87 @@ __device__ __forceinline__ void load_fixed(Lds &S) {
   build_table(S.lens, 32, kTabDist, S.hd, S.dist, kDistBits);
 }
 
-// Decodes symbols until end-of-block. Returns a status.
-__device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, Lds &S) {
+// Symbol loop: decodes up to `limit` (<= 64) symbols of the current block
+// into tok (lane j <- token j, nt tokens). Returns kOk when the batch is full
+// (`limit` tokens, or the batch's matches reached kBatchBytes), kEob at
+// end-of-block, or the status of an invalid symbol. The tokens decoded before
+// an invalid symbol are valid and still have to be written. All stops leave
+// through one exit so the loop's uniform state stays in scalar registers.
+__device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t lane, uint32_t &tok,
+                                                 uint32_t &nt, uint32_t limit) {
+  uint32_t bb = 0;
+  int32_t st = kOk;
   for (;;) {
     bits_refill(b);
-    uint32_t e = uni(S.lit[b.buf & ((1u << kLitBits) - 1)]);
+    uint32_t e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
+    if (e & kLitFlag) {  // literal: the token is the table entry shifted up
+      bits_drop(b, e & 15);
+      if (lane == nt) tok = e << 16;
+      if (++nt == limit) break;
+      continue;
+    }
     uint32_t L = e & 15;
-    if (L == 0) {
-      int32_t sym = slow_decode(S.hl, b.buf, &L);
-      if (sym < 0) return bits_past_end(b) ? kTrunc : kCorrupt;
+    if (L == 0) {  // code longer than the LUT, or no such code
+      const int32_t sym = slow_decode(S.hl, b.buf, &L);
+      if (sym < 0) {
+        st = bits_past_end(b) ? kTrunc : kCorrupt;
+        break;
+      }
       e = make_entry(kTabLit, (uint32_t)sym, L);
     }
     bits_drop(b, L);
-    const uint32_t kind = (e >> 4) & 15;
-    if (kind == kKindLit) {
-      if (bits_past_end(b)) return kTrunc;
-      if (o.pos >= o.cap) return kOverflow;
-      out_literal(o, (e >> 8) & 0xff);
+    if (e & kLitFlag) {
+      if (lane == nt) tok = e << 16;
+      if (++nt == limit) break;
       continue;
     }
-    if (kind == kKindEob) return bits_past_end(b) ? kTrunc : kOk;
-    if (kind == kKindBad) return kCorrupt;
-    // length
-    const uint32_t lx = (e >> 20) & 15;
-    const uint32_t len = ((e >> 8) & 0x1ff) + (uint32_t)(b.buf & ((1u << lx) - 1));
+    const uint32_t kind = (e >> 12) & 3;
+    if (kind != kKindLen) {
+      st = kind == kKindEob ? kEob : kCorrupt;
+      break;
+    }
+    const uint32_t lx = (e >> 4) & 7;
+    const uint32_t len = len_base((e >> 7) & 31, lx) + ((uint32_t)b.buf & ((1u << lx) - 1));
     bits_drop(b, lx);
     bits_refill(b);
-    uint32_t d = uni(S.dist[b.buf & ((1u << kDistBits) - 1)]);
+    uint32_t d = uni(S.dist[(uint32_t)b.buf & kDistMask]);
     uint32_t dl = d & 15;
     if (dl == 0) {
-      int32_t sym = slow_decode(S.hd, b.buf, &dl);
-      if (sym < 0) return bits_past_end(b) ? kTrunc : kCorrupt;
+      const int32_t sym = slow_decode(S.hd, b.buf, &dl);
+      if (sym < 0) {
+        st = bits_past_end(b) ? kTrunc : kCorrupt;
+        break;
+      }
       d = make_entry(kTabDist, (uint32_t)sym, dl);
     }
-    if ((d >> 4) & 15) return kCorrupt;  // distance code 30/31
+    if (d & kDistBad) {  // distance codes 30 and 31
+      st = kCorrupt;
+      break;
+    }
     bits_drop(b, dl);
-    const uint32_t dx = (d >> 24) & 15;
-    const uint32_t dist = ((d >> 8) & 0xffff) + (uint32_t)(b.buf & ((1u << dx) - 1));
+    const uint32_t dx = (d >> 4) & 15;
+    const uint32_t dist = dist_base((d >> 8) & 31, dx) + ((uint32_t)b.buf & ((1u << dx) - 1));
     bits_drop(b, dx);
+    if (lane == nt) tok = len | (dist << 9);
+    bb += len;
+    if (++nt == limit || bb >= kBatchBytes) break;
+  }
+  return st;
+}
+
+// Decodes symbols until end-of-block. Returns a status.
+__device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, uint32_t lane) {
+  for (;;) {
+    // Near the stream end a batch is one symbol, so reading past the end is
+    // caught at the symbol that does it.
+    const bool careful = bits_pos(b) + kCarefulBits > b.limit;
+    uint32_t tok = 0, nt = 0;
+    const int32_t sa = decode_tokens(b, S, lane, tok, nt, careful ? 1u : (uint32_t)kWave);
     if (bits_past_end(b)) return kTrunc;
-    if (dist > o.pos) return kCorrupt;  // zlib: "invalid distance too far back"
-    if (o.pos + len > o.cap) return kOverflow;
-    out_match(o, dist, len);
+    const int32_t sb = out_tokens(o, lane, tok, nt);
+    if (sb != kOk) return sb;
+    if (sa == kEob) return kOk;
+    if (sa != kOk) return sa;
   }
 }
 
@@ -620,15 +690,7 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
   o.cap = (uint32_t)cap;
   o.pos = 0;
   o.flushed = 0;
-  o.retired = 0;
   o.mis = (uint32_t)(reinterpret_cast<uintptr_t>(dst) & 15);
-  o.litbuf = 0;
-  o.nlit = 0;
-  o.np = 0;
-  o.pdst = o.psrc = o.plen = 0;
-  o.hole_lo = 0xffffffffu;
-  o.hole_hi = 0;
-  o.src_hi = 0;
   o.s1 = 0;
   o.s2 = 0;
 
@@ -653,14 +715,14 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
       const uint64_t at = bits_pos(b) / 8 - lead;  // stream byte offset of the data
       if (at + ln > len) { st = kTrunc; break; }
       if (o.pos + ln > o.cap) { st = kOverflow; break; }
-      out_lits_to_ring(o);
       for (uint32_t done = 0; done < ln;) {  // through the ring, <= 256 bytes per step
         const uint32_t step = (ln - done) < 256u ? (ln - done) : 256u;
+#pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
         for (uint32_t i = lane; i < step; i += kWave)
           o.ring[ridx(o, o.pos + i)] = src[at + done + i];
         o.pos += step;
         done += step;
-        out_maybe_flush(o);
+        if (o.pos - o.flushed >= kFlushMin) out_store(o, lane, o.pos, false);
       }
       bits_seek(b, lead + at + ln);
     } else if (type == 1) {  // fixed Huffman
@@ -668,29 +730,26 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
         load_fixed(S);
         tables = 1;
       }
-      st = decode_block(b, o, S);
+      st = decode_block(b, o, S, lane);
     } else if (type == 2) {  // dynamic Huffman
       uint32_t nlen = 0;
       tables = 0;
       st = read_dynamic(b, S, &nlen);
-      if (st == kOk) st = decode_block(b, o, S);
+      if (st == kOk) st = decode_block(b, o, S, lane);
     } else {
       st = kCorrupt;  // zlib: "invalid block type"
     }
-    out_reduce_mod(o);
   }
-  // drain: pending literals and far pieces, then the last partial flush
-  out_lits_to_ring(o);
-  out_resolve(o);
-  __syncthreads();
-  out_store(o, o.pos, true);
-  // wave reduction of the Adler-32 partial sums
-  uint64_t s1 = o.s1 % 65521u, s2 = o.s2 % 65521u;
+  // drain: the last partial flush
+  out_store(o, lane, o.pos, true);
+  // wave reduction of the Adler-32 partial sums (each already < 65521)
+  uint32_t s1r = o.s1, s2r = o.s2;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
-    s1 += __shfl_xor(s1, d, kWave);
-    s2 += __shfl_xor(s2, d, kWave);
+    s1r += __shfl_xor(s1r, d, kWave);
+    s2r += __shfl_xor(s2r, d, kWave);
   }
+  uint64_t s1 = s1r, s2 = s2r;
   s1 %= 65521u;
   s2 %= 65521u;
   const uint64_t n = o.pos % 65521u;
@@ -714,7 +773,7 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
   return res;
 }
 
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMinWavesPerEu)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
 inflate_chunks_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                       const uint64_t *__restrict__ off, uint64_t n, uint64_t chunk_bytes, uint8_t *__restrict__ out,
                       uint64_t *__restrict__ out_len, int32_t *__restrict__ status, uint32_t flags) {
@@ -733,7 +792,7 @@ inflate_chunks_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   }
 }
 
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMinWavesPerEu)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
 inflate_batch_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
                      codag_stream_result *__restrict__ results, uint32_t flags,
                      uint32_t *__restrict__ counter) {

@@ -40,4 +40,8 @@ hipError_t launch_inflate_batch(const codag_stream_desc *descs, uint64_t n,
                                 codag_stream_result *results, uint32_t flags, uint32_t *counter,
                                 uint32_t groups, hipStream_t stream);
 
+// Plain device copy (16-byte loads/stores): the reference point for HBM
+// traffic counters.
+hipError_t launch_copy(uint8_t *dst, const uint8_t *src, uint64_t bytes, hipStream_t stream);
+
 }  // namespace codag

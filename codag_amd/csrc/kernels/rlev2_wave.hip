@@ -74,20 +74,37 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   uint8_t *dst = out + chunk * chunk_bytes;
   const uint32_t *sdw = reinterpret_cast<const uint32_t *>(L.sbuf);
 
-  uint64_t wpos = 0;  // stream offset held at sbuf[lead]
+  // Stream offsets are 32-bit here: a chunk's stream is far below 4 GiB.
+  uint32_t wpos = 0;  // stream offset held at sbuf[lead]
   uint32_t lead = 0, P = 0;
   bool at_end = true;
-  auto stage = [&](uint64_t pos) {
+  // The walk stops at p >= pdone (stream consumed: P when the window holds the
+  // stream's end) and restages at p > pstop (header bytes might lie past the
+  // window: P - 32 when it does not).
+  uint32_t pdone = 0, pstop = 0;
+  // SHORT_REPEAT fast path bounds (a block is at most 10 bytes and 10 values):
+  // p < pfast keeps the header and the whole block inside the staged stream,
+  // o < ofast keeps the values inside the chunk.
+  uint32_t pfast = 0;
+  const uint32_t ofast = cap >= 10 ? cap - 9 : 0;
+  auto stage = [&](uint32_t pos) {
     __syncthreads();  // earlier LDS reads of this wave are done before overwrite
-    const uint32_t n = (uint32_t)((len - pos) < (kWin - 16) ? (len - pos) : (kWin - 16));
+    const uint32_t n = ((uint32_t)len - pos) < (kWin - 16) ? ((uint32_t)len - pos) : (kWin - 16);
     lead = 0;
     if (n) stage_to_lds(L.sbuf, src, pos, n, lane, kWave, &lead);
     P = lead + n;
-    at_end = (pos + n == len);
+    at_end = (pos + n == (uint32_t)len);
     wpos = pos;
+    pdone = at_end ? P : 0xffffffffu;
+    pstop = at_end ? 0xffffffffu : P - 32;
+    pfast = at_end ? (P >= 10 ? P - 9 : 0) : P - 31;
     __syncthreads();
   };
 
+  if (len >= 0xffffffffull) {  // outside the 32-bit stream offsets used below
+    if (lane == 0 && status) status[chunk] = kCorrupt;
+    return;
+  }
   stage(0);
   uint32_t p = lead, o = 0;
   uint32_t vbase = p & ~3u;
@@ -100,11 +117,35 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
     uint32_t nd = 0, dpos = 0, dout = 0;
     bool restage = false;
     while (nd < (uint32_t)kWave) {
-      if (wpos + (p - lead) >= len) {  // decoder_new.h:1000 -- stream consumed
+      // Fast path for runs of SHORT_REPEAT blocks (the most frequent header):
+      // while the position and output index are far enough from the window
+      // end and the chunk end that no bound can be hit, a block costs one
+      // header read, two bit-field extracts and the lane record.
+      for (;;) {
+        if (nd >= (uint32_t)kWave || p >= pfast || o >= ofast) break;
+        if (p - vbase > 224) {
+          vbase = p & ~3u;
+          vw = sdw[vbase / 4 + lane];
+        }
+        const uint32_t k = (p - vbase) >> 2;
+        const uint32_t first = (uint32_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k) |
+                                           ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k + 1)
+                                            << 32)) >> ((p & 3) * 8)) & 0xff;
+        if (first >= 64) break;
+        if (lane == nd) {
+          dpos = p;
+          dout = o;
+        }
+        nd++;
+        o += (first & 7) + 3;
+        p += 2 + ((first >> 3) & 7);
+      }
+      if (nd >= (uint32_t)kWave) break;
+      if (p >= pdone) {  // decoder_new.h:1000 -- stream consumed
         done = true;
         break;
       }
-      if (!at_end && p + 32 > P) {  // header bytes might lie past the window
+      if (p > pstop) {  // header bytes might lie past the window
         restage = true;
         break;
       }
@@ -112,14 +153,18 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
         vbase = p & ~3u;
         vw = sdw[vbase / 4 + lane];
       }
-      const uint64_t x = win8(vw, vbase, p);
-      const uint32_t first = (uint32_t)(x & 0xff);
-      const uint32_t kind = first >> 6;
+      // four header bytes from two adjacent dwords of the VGPR mirror
+      const uint32_t k = (p - vbase) >> 2;
+      const uint64_t pair = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k) |
+                            ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k + 1) << 32);
+      const uint32_t x = (uint32_t)(pair >> ((p & 3) * 8));
+      const uint32_t first = x & 0xff;
       uint32_t size, cnt;
-      if (kind == 0) {  // SHORT_REPEAT
+      if (first < 64) {  // SHORT_REPEAT
         size = 2 + ((first >> 3) & 7);
         cnt = (first & 7) + 3;
       } else {
+        const uint32_t kind = first >> 6;
         const uint32_t code = (first >> 1) & 0x1f;
         const uint32_t fbw = decode_width(code);
         const uint32_t len8 = ((((first & 1) << 8) | (uint32_t)((x >> 8) & 0xff)) + 1) & 0xff;

@@ -331,6 +331,10 @@ codag_status codag_dev_free(codag_ctx *ctx, void *d_ptr);
 codag_status codag_copy_h2d(codag_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
 codag_status codag_copy_d2h(codag_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 codag_status codag_memset_d(codag_ctx *ctx, void *d_dst, int value, uint64_t bytes);
+/* Device-to-device copy by a plain 16-byte-vector kernel on the context
+ * stream (asynchronous); both pointers 16-byte aligned. Used as the
+ * reference point when HBM traffic counters are scaled. */
+codag_status codag_copy_d2d(codag_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);
 codag_status codag_sync(codag_ctx *ctx);
 /* HIP events recorded on the context stream around a region. */
 codag_status codag_timer_start(codag_ctx *ctx);
