@@ -50,7 +50,10 @@ def crc32(b: np.ndarray) -> int:
 
 
 def traffic_from_profiles(kernel_key: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any.
+
+    Keys are "<kernel>:<config>:<workgroups>"; the bytes come from FETCH_SIZE
+    and WRITE_SIZE scaled by the same counters of the plain copy kernel."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -74,8 +77,20 @@ class Column:
         self.d_st = ctx.alloc(4 * n)
         self.host = (comp, blk[:n].copy(), cl[:n].copy())
         self.alg_bytes = len(comp) + n * CH + 16 * n
+        self.tune = None  # launch shape chosen by the autotuner for this column
+
+    def autotune(self, ctx):
+        self.tune, ms = ctx.tune_run(self.codec, self.d_comp, self.comp_bytes, self.d_off,
+                                     self.d_len, self.n, self.width, CH, self.d_out)
+        return ms
+
+    def shape(self):
+        t = self.tune
+        return None if t is None else [t.threads_per_group, t.window_bytes, t.queue_depth]
 
     def launch(self, ctx):
+        if self.tune is not None:  # host-side parameter switch, no device work
+            ctx.tune_set(self.codec, self.tune)
         fn = ctx.rlev1_decode if self.codec == ca.RLEV1 else ctx.rlev2_decode
         fn(self.d_comp, self.comp_bytes, self.d_off, self.d_len, self.n, self.width, CH,
            self.d_out, self.d_st)
@@ -131,6 +146,11 @@ def bench_rle(ctx, args, cfg):
     log(f"[bench] generating workload {cfg}")
     cols, desc, dtype = make_rle(ctx, cfg)
     out_bytes = sum(len(c.raw) for c in cols)
+    if not args.no_autotune:
+        for c in cols:
+            ms = c.autotune(ctx)
+            log(f"[bench] autotuner: {c.width * 8}-bit column -> launch shape {c.shape()} "
+                f"({ms:.3f} ms)")
     for c in cols:
         c.launch(ctx)
     ctx.sync()
@@ -180,12 +200,17 @@ def bench_rle(ctx, args, cfg):
     achieved = cols[dom].alg_bytes / (k_ms / 1e3) / 1e9
     if cols[dom].codec == ca.RLEV1:
         kname = "rlev1_decode_kernel"
-    else:  # a 64-thread launch shape is the one-wave-per-chunk kernel
-        kname = ("rlev2_wave_kernel" if ctx.tune_get(ca.RLEV2).threads_per_group == 64
-                 else "rlev2_decode_kernel")
+    else:  # the launch shape selects the kernel
+        t = cols[dom].tune or ctx.tune_get(ca.RLEV2)
+        if t.queue_depth == 2 and t.threads_per_group in (128, 256):
+            kname = "rlev2_pc_kernel"
+        elif t.threads_per_group == 64:
+            kname = "rlev2_wave_kernel"
+        else:
+            kname = "rlev2_decode_kernel"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(f"{kname}:{cfg}"),
+            "traffic": traffic_from_profiles(f"{kname}:{cfg}:{cols[dom].n}"),
             "kernel": kname, "kernel_ms": round(k_ms, 4),
             "alg_bytes_per_launch": int(cols[dom].alg_bytes)}
     cpu = None if args.no_cpu else cpu_rle_baseline(cols)
@@ -197,7 +222,8 @@ def bench_rle(ctx, args, cfg):
         "config": {"workload": desc, "chunks": [c.n for c in cols],
                    "compressed_bytes": [c.comp_bytes for c in cols],
                    "decompressed_bytes": out_bytes, "crc32": crcs,
-                   "wall_s": round(wall, 3)},
+                   "wall_s": round(wall, 3),
+                   "launch_shapes": [c.shape() for c in cols]},
         "roofline": roof, "cpu_baseline": cpu,
     }
 
@@ -265,7 +291,7 @@ def bench_deflate(ctx, args):
                    "compressed_bytes": int(len(comp)), "crc32": crc},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("inflate_chunks_kernel:c3"),
+                     "traffic": traffic_from_profiles(f"inflate_chunks_kernel:c3:{n}"),
                      "kernel": "inflate_chunks_kernel", "kernel_ms": round(ms, 4),
                      "alg_bytes_per_launch": int(alg)},
         "cpu_baseline": cpu,
@@ -516,6 +542,8 @@ def main():
     ap.add_argument("--streams", type=int, default=10000, help="c4: number of zlib streams")
     ap.add_argument("--requests", type=int, default=48, help="c5: requests per load level")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="c1/c2: keep the default launch shape instead of tuning per column")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="first run the plain copy kernel over MIB MiB (traffic calibration)")
     args = ap.parse_args()

@@ -52,12 +52,14 @@ enum : uint32_t { kTabLit = 0, kTabDist = 1, kTabCodes = 2 };
 
 // 16-bit LUT entries (bits 3:0 always hold the code length; 0 = not in LUT):
 //   literal      : bit 15 set, byte in bits 11:4
-//   length code c: extra-bit count in bits 6:4, c = symbol-257 in bits 11:7,
-//                  kind (kKindLen / kKindEob / kKindBad) in bits 13:12
-//   distance d   : extra-bit count in bits 7:4, d in bits 12:8, bit 13 = invalid (d >= 30)
+//   length code c: extra-bit count in bits 6:4 (0..5), base-3 in bits 14:7;
+//                  bits 6:4 = 7 marks end-of-block, 6 an invalid symbol
+//   distance d   : extra-bit count in bits 7:4, m in bits 9:8 with
+//                  base = (m << extra) + 1; bit 13 = invalid (d >= 30)
 //   code-length  : symbol in bits 12:8
 constexpr uint32_t kLitFlag = 0x8000u;
 constexpr uint32_t kDistBad = 0x2000u;
+constexpr uint32_t kXEob = 7, kXBad = 6;
 
 // RFC 1951 section 3.2.5 length and distance codes (the tables of
 // include/zlib/zlib.h:35-51) in closed form: length code c (0..28) has
@@ -311,17 +313,17 @@ __device__ __forceinline__ void gather_far(Out &o, uint32_t lane, uint64_t fm, u
 }
 
 // Materialises a batch of nt tokens (lane j holds token j):
-//   literal: bit 31 set, byte in bits 27:20 (its LUT entry << 16);
-//   match: length in bits 8:0, distance in bits 24:9.
+//   literal: its LUT entry (bit 15 set, byte in bits 11:4);
+//   match: length in bits 8:0 (bit 15 clear), distance in bits 31:16.
 // Errors are reported for the first offending token, in stream order:
 // output past cap (overflow) or a distance beyond the bytes produced so far.
 __device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t tok, uint32_t nt) {
   if (nt == 0) return kOk;
   const bool act = lane < nt;
-  const bool lit = act && (int32_t)tok < 0;
-  const bool mat = act && (int32_t)tok >= 0;
+  const bool lit = act && (tok & kLitFlag);
+  const bool mat = act && !(tok & kLitFlag);
   const uint32_t mlen = tok & 0x1ff;
-  const uint32_t dist = (tok >> 9) & 0xffff;
+  const uint32_t dist = tok >> 16;
   const uint32_t tl = lit ? 1u : (mat ? mlen : 0u);
   const uint32_t incl = wave_incl_scan(tl);
   const uint32_t x = o.pos + incl - tl;  // output position of this token
@@ -332,7 +334,7 @@ __device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t to
   if (em) return (int32_t)rdlane(err, (uint32_t)__builtin_ctzll(em));
   const uint32_t pend = o.pos + rdlane(incl, nt - 1);
 
-  if (lit) o.ring[ridx(o, x)] = (uint8_t)(tok >> 20);
+  if (lit) o.ring[ridx(o, x)] = (uint8_t)(tok >> 4);
   // a source byte y is intact in the ring iff y >= pend - kRing
   const bool far = mat && (x - dist) + kRing < pend;
   const uint64_t fm = __ballot(far);
@@ -343,10 +345,14 @@ __device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t to
     nm &= nm - 1;
     const uint32_t xj = rdlane(x, j), dj = rdlane(dist, j), lj = rdlane(mlen, j);
     const uint32_t sj = xj - dj;
-    if (dj >= (uint32_t)kWave) {
-      // byte i >= dj repeats byte i - dj, written one 64-byte step earlier
+    if (dj >= (uint32_t)kWave || lj <= dj) {
+      // byte i reads source byte sj + i: either the source ends before the
+      // match (lj <= dj) or, for i >= dj, it repeats byte i - dj, written one
+      // 64-byte step earlier. Most matches take the single first step.
+      if (lane < lj) o.ring[ridx(o, xj + lane)] = o.ring[ridx(o, sj + lane)];
 #pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
-      for (uint32_t i = lane; i < lj; i += kWave) o.ring[ridx(o, xj + i)] = o.ring[ridx(o, sj + i)];
+      for (uint32_t i = kWave + lane; i < lj; i += kWave)
+        o.ring[ridx(o, xj + i)] = o.ring[ridx(o, sj + i)];
     } else {
       // periodic copy: byte i = source byte i mod dj. The float quotient is
       // exact up to one unit for i < 2^9, dj < 64; the remainder is corrected.
@@ -372,15 +378,16 @@ __device__ __forceinline__ uint32_t make_entry(uint32_t tab, uint32_t sym, uint3
   if (tab == kTabCodes) return len | (sym << 8);
   if (tab == kTabDist) {
     if (sym >= 30) return len | kDistBad;
-    return len | (dist_extra(sym) << 4) | (sym << 8);
+    const uint32_t m = sym < 4 ? sym : 2 + (sym & 1);  // dist_base == (m << extra) + 1
+    return len | (dist_extra(sym) << 4) | (m << 8);
   }
   if (sym < 256) return len | (sym << 4) | kLitFlag;
-  if (sym == 256) return len | (kKindEob << 12);
+  if (sym == 256) return len | (kXEob << 4);
   if (sym <= 285) {
-    const uint32_t c = sym - 257;
-    return len | (len_extra(c) << 4) | (c << 7) | (kKindLen << 12);
+    const uint32_t c = sym - 257, x = len_extra(c);
+    return len | (x << 4) | ((len_base(c, x) - 3) << 7);
   }
-  return len | (kKindBad << 12);
+  return len | (kXBad << 4);
 }
 
 // Builds the canonical tables for n code lengths in lens[] and the LUT of
@@ -577,14 +584,23 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
   uint32_t bb = 0;
   int32_t st = kOk;
   for (;;) {
-    bits_refill(b);
+    bits_refill(b);  // >= 33 bits
     uint32_t e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
-    if (e & kLitFlag) {  // literal: the token is the table entry shifted up
+    if (e & kLitFlag) {  // literal: the token is the table entry
       bits_drop(b, e & 15);
-      if (lane == nt) tok = e << 16;
+      if (lane == nt) tok = e;
       if (++nt == limit) break;
-      continue;
+      // >= 18 bits are left: enough for the next code without a refill
+      e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
+      if (e & kLitFlag) {
+        bits_drop(b, e & 15);
+        if (lane == nt) tok = e;
+        if (++nt == limit) break;
+        continue;
+      }
     }
+    // a length / end-of-block / invalid code, or one longer than the LUT;
+    // the buffer holds >= 18 bits
     uint32_t L = e & 15;
     if (L == 0) {  // code longer than the LUT, or no such code
       const int32_t sym = slow_decode(S.hl, b.buf, &L);
@@ -596,19 +612,20 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
     }
     bits_drop(b, L);
     if (e & kLitFlag) {
-      if (lane == nt) tok = e << 16;
+      if (lane == nt) tok = e;
       if (++nt == limit) break;
       continue;
     }
-    const uint32_t kind = (e >> 12) & 3;
-    if (kind != kKindLen) {
-      st = kind == kKindEob ? kEob : kCorrupt;
+    const uint32_t lx = (e >> 4) & 7;
+    if (lx >= kXBad) {
+      st = lx == kXEob ? kEob : kCorrupt;
       break;
     }
-    const uint32_t lx = (e >> 4) & 7;
-    const uint32_t len = len_base((e >> 7) & 31, lx) + ((uint32_t)b.buf & ((1u << lx) - 1));
-    bits_drop(b, lx);
+    // >= 33 bits after the refill: length extra (<= 5), distance code (<= 15)
+    // and distance extra (<= 13)
     bits_refill(b);
+    const uint32_t len = ((e >> 7) & 255) + 3 + ((uint32_t)b.buf & ((1u << lx) - 1));
+    bits_drop(b, lx);
     uint32_t d = uni(S.dist[(uint32_t)b.buf & kDistMask]);
     uint32_t dl = d & 15;
     if (dl == 0) {
@@ -625,9 +642,9 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
     }
     bits_drop(b, dl);
     const uint32_t dx = (d >> 4) & 15;
-    const uint32_t dist = dist_base((d >> 8) & 31, dx) + ((uint32_t)b.buf & ((1u << dx) - 1));
+    const uint32_t dist = (((d >> 8) & 3) << dx) + 1 + ((uint32_t)b.buf & ((1u << dx) - 1));
     bits_drop(b, dx);
-    if (lane == nt) tok = len | (dist << 9);
+    if (lane == nt) tok = len | (dist << 16);
     bb += len;
     if (++nt == limit || bb >= kBatchBytes) break;
   }

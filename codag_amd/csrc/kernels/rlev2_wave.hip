@@ -1,17 +1,24 @@
-// rlev2_wave.hip -- RLE v2 decode with one self-contained wave64 per chunk.
+// rlev2_wave.hip -- RLE v2 decode kernels built around a scalar header walk.
 //
-// The chunk's stream is staged into a per-wave LDS window (kWin bytes). The
-// header walk runs in scalar registers: the 256 bytes around the walk
-// position are mirrored in one VGPR (lane l holds dword l) and header bytes
-// are read with v_readlane, so walking a block costs no memory round trip.
-// The walk collects up to 64 blocks (lane j <- block j), then:
+// The chunk's stream is staged into an LDS window (kWin bytes). The header
+// walk runs in scalar registers: the 256 bytes around the walk position are
+// mirrored in one VGPR (lane l holds dword l) and header bytes are read with
+// v_readlane, so walking a block costs no memory round trip. A walk collects
+// up to 64 blocks (lane j <- block j); expansion then writes them:
 //   * SHORT_REPEAT blocks (3..10 values) expand one lane per block;
-//   * DIRECT / DELTA / PATCHED_BASE blocks expand across the wave
-//     (expand_big, one lane per value, wave prefix sums for DELTA, a wave
-//     scan of the patch gaps for PATCHED_BASE).
-// A block that does not fit in the window restarts the window at that
-// block. There are no workgroup barriers: the chunk's progress depends only
-// on its own wave, and several chunks share each SIMD to hide latency.
+//   * DIRECT / DELTA / PATCHED_BASE blocks expand across a wave (expand_big:
+//     one lane per value, wave prefix sums for DELTA, a wave scan of the
+//     patch gaps for PATCHED_BASE).
+// A block that does not fit in the window restarts the window at that block.
+//
+// Two kernels share the walk and the expansion:
+//   * rlev2_wave_kernel: one wave per chunk walks and expands in turn, with no
+//     workgroup barriers; several chunks share each SIMD to hide latency.
+//   * rlev2_pc_kernel: one walker wave and kExp expander waves per chunk. The
+//     walker publishes each batch of block descriptors to an LDS double buffer
+//     and walks the next batch while the expanders write the previous one, so
+//     a chunk's time is max(walk, expand) instead of their sum and a chunk
+//     keeps more waves busy (the columns with few chunks gain most).
 //
 // Semantics: include/rlev2/decoder_new.h:909-1187 (decoder_warp_orig_rdw).
 #include "kernels.hpp"
@@ -48,6 +55,192 @@ __device__ __forceinline__ uint32_t varint_len(const uint8_t *sbuf, uint32_t vw,
   return n + 1;
 }
 
+// Walk state of one chunk. Stream offsets are 32-bit: a chunk's stream is far
+// below 4 GiB (longer streams are rejected up front).
+struct Walk {
+  const uint8_t *src;  // chunk stream
+  uint32_t len;        // stream bytes
+  uint32_t cap;        // values in the chunk
+  uint32_t ofast;      // o < ofast keeps a SHORT_REPEAT block inside the chunk
+  uint32_t wpos;       // stream offset held at sbuf[lead]
+  uint32_t lead, P;    // window: bytes [lead, P) of sbuf hold the stream
+  bool at_end;         // the window holds the stream's end
+  // The walk stops at p >= pdone (stream consumed: P when the window holds
+  // the stream's end), restages at p > pstop (header bytes might lie past
+  // the window: P - 32 when it does not), and takes the SHORT_REPEAT fast
+  // path while p < pfast (a block is at most 10 bytes).
+  uint32_t pdone, pstop, pfast;
+  uint32_t p, o;       // walk position (LDS offset) and values produced
+  uint32_t vbase, vw;  // VGPR mirror of LDS bytes [vbase, vbase + 256)
+};
+
+// Stages stream bytes [pos, ...) into the window with the 64 lanes of the
+// calling wave and restarts the walk at pos. The caller orders it against
+// other waves' reads of the window.
+template <uint32_t kWin>
+__device__ __forceinline__ void walk_stage(Walk &s, uint8_t *sbuf, uint32_t pos, uint32_t lane) {
+  const uint32_t n = (s.len - pos) < (kWin - 16) ? (s.len - pos) : (kWin - 16);
+  s.lead = 0;
+  if (n) stage_to_lds(sbuf, s.src, pos, n, lane, kWave, &s.lead);
+  s.P = s.lead + n;
+  s.at_end = (pos + n == s.len);
+  s.wpos = pos;
+  s.pdone = s.at_end ? s.P : 0xffffffffu;
+  s.pstop = s.at_end ? 0xffffffffu : s.P - 32;
+  s.pfast = s.at_end ? (s.P >= 10 ? s.P - 9 : 0) : s.P - 31;
+  s.p = s.lead;
+}
+
+// Reloads the VGPR mirror at the walk position (after the window changed).
+__device__ __forceinline__ void walk_mirror(Walk &s, const uint8_t *sbuf, uint32_t lane) {
+  s.vbase = s.p & ~3u;
+  s.vw = reinterpret_cast<const uint32_t *>(sbuf)[s.vbase / 4 + lane];
+}
+
+// Collects up to 64 blocks: lane j receives block j's LDS offset (dpos) and
+// first value index (dout). Sets *done when the stream is consumed, *restage
+// when the next header or block may lie past the window, *st on an error.
+__device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uint32_t lane,
+                                               uint32_t &dpos, uint32_t &dout, bool &done,
+                                               bool &restage, int32_t &st) {
+  const uint32_t *sdw = reinterpret_cast<const uint32_t *>(sbuf);
+  uint32_t nd = 0;
+  uint32_t p = s.p, o = s.o, vbase = s.vbase, vw = s.vw;
+  while (nd < (uint32_t)kWave) {
+    // Fast path for runs of SHORT_REPEAT blocks (the most frequent header):
+    // while the position and output index are far enough from the window
+    // end and the chunk end that no bound can be hit, a block costs one
+    // header read, two bit-field extracts and the lane record.
+    for (;;) {
+      if (nd >= (uint32_t)kWave || p >= s.pfast || o >= s.ofast) break;
+      if (p - vbase > 224) {
+        vbase = p & ~3u;
+        vw = sdw[vbase / 4 + lane];
+      }
+      const uint32_t k = (p - vbase) >> 2;
+      const uint32_t first = (uint32_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k) |
+                                         ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k + 1)
+                                          << 32)) >> ((p & 3) * 8)) & 0xff;
+      if (first >= 64) break;
+      if (lane == nd) {
+        dpos = p;
+        dout = o;
+      }
+      nd++;
+      o += (first & 7) + 3;
+      p += 2 + ((first >> 3) & 7);
+    }
+    if (nd >= (uint32_t)kWave) break;
+    if (p >= s.pdone) {  // decoder_new.h:1000 -- stream consumed
+      done = true;
+      break;
+    }
+    if (p > s.pstop) {  // header bytes might lie past the window
+      restage = true;
+      break;
+    }
+    if (p - vbase > 224) {
+      vbase = p & ~3u;
+      vw = sdw[vbase / 4 + lane];
+    }
+    // four header bytes from two adjacent dwords of the VGPR mirror
+    const uint32_t k = (p - vbase) >> 2;
+    const uint64_t pair = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k) |
+                          ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k + 1) << 32);
+    const uint32_t x = (uint32_t)(pair >> ((p & 3) * 8));
+    const uint32_t first = x & 0xff;
+    uint32_t size, cnt;
+    if (first < 64) {  // SHORT_REPEAT
+      size = 2 + ((first >> 3) & 7);
+      cnt = (first & 7) + 3;
+    } else {
+      const uint32_t kind = first >> 6;
+      const uint32_t code = (first >> 1) & 0x1f;
+      const uint32_t fbw = decode_width(code);
+      const uint32_t len8 = ((((first & 1) << 8) | (uint32_t)((x >> 8) & 0xff)) + 1) & 0xff;
+      if (kind == 1) {  // DIRECT
+        size = 2 + (len8 * fbw + 7) / 8;
+        cnt = len8;
+      } else if (kind == 3) {  // DELTA: two varints then packed deltas
+        const uint32_t nd8 = (len8 - 2) & 0xff;
+        uint32_t q = p + 2;
+        const uint32_t l1 = varint_len(sbuf, vw, vbase, q, s.P);
+        const uint32_t l2 = q + l1 < s.P ? varint_len(sbuf, vw, vbase, q + l1, s.P) : 1;
+        if (q + l1 + l2 <= s.P && (l1 > 10 || l2 > 10)) {  // longer than any 64-bit varint
+          st = kCorrupt;
+          break;
+        }
+        q += l1 + l2;
+        size = (q - p) + (code ? (nd8 * fbw + 7) / 8 : 0);
+        cnt = 2 + nd8;
+      } else {  // PATCHED_BASE
+        const uint32_t third = (x >> 16) & 0xff, fourth = (x >> 24) & 0xff;
+        const uint32_t pw = decode_width(third & 0x1f);
+        const uint32_t pgw = (fourth >> 5) + 1;
+        if (pw + pgw > 64) {  // beyond the 65-entry closest-width table
+          st = kCorrupt;
+          break;
+        }
+        size = 4 + (third >> 5) + 1 + (len8 * fbw + 7) / 8 +
+               ((fourth & 0x1f) * closest_fixed(pw + pgw) + 7) / 8;
+        cnt = len8;
+      }
+    }
+    if (p + size > s.P) {
+      if (s.at_end) st = kTrunc;
+      else restage = true;
+      break;
+    }
+    if (o + cnt > s.cap) {
+      st = kOverflow;
+      break;
+    }
+    if (lane == nd) {
+      dpos = p;
+      dout = o;
+    }
+    nd++;
+    o += cnt;
+    p += size;
+  }
+  s.p = p;
+  s.o = o;
+  s.vbase = vbase;
+  s.vw = vw;
+  return nd;
+}
+
+// Expands the blocks of a batch (lane j holds block j's dpos/dout, j < nd)
+// that `sel` selects. All 64 lanes of the wave must call it.
+__device__ __forceinline__ void expand_batch(const uint8_t *sbuf, uint32_t nd, uint64_t sel,
+                                             uint32_t dpos, uint32_t dout, uint8_t *dst, uint32_t w,
+                                             uint64_t mask, uint32_t nbits, uint32_t lane,
+                                             uint32_t *bad) {
+  const bool mine = lane < nd && ((sel >> lane) & 1);
+  const uint32_t myfirst = mine ? sbuf[dpos] : 0u;
+  const bool is_sr = mine && (myfirst >> 6) == 0;
+  if (is_sr) {  // SHORT_REPEAT: big-endian value, 3..10 copies (decoder_new.h:1003-1015)
+    const uint32_t nb = ((myfirst >> 3) & 7) + 1;
+    const uint32_t c = (myfirst & 7) + 3;
+    const uint64_t v = lds_be(sbuf + dpos + 1, nb) & mask;
+    for (uint32_t k = 0; k < c; k++) store_elem(dst + (uint64_t)(dout + k) * w, w, v);
+  }
+  uint64_t big = __ballot(mine && !is_sr);
+  while (big) {
+    const uint32_t j = (uint32_t)__builtin_ctzll(big);
+    big &= big - 1;
+    const uint32_t bp = (uint32_t)__builtin_amdgcn_readlane(dpos, j);
+    const uint32_t bo = (uint32_t)__builtin_amdgcn_readlane(dout, j);
+    expand_big(sbuf, bp, bo, dst, w, mask, nbits, lane, bad);
+  }
+}
+
+// Validates the chunk's metadata; false (status written) when the stream
+// lies outside comp or exceeds the 32-bit offsets.
+__device__ __forceinline__ bool chunk_ok(uint64_t boff, uint64_t len, uint64_t comp_bytes) {
+  return boff <= comp_bytes && len <= comp_bytes - boff && len < 0xffffffffull;
+}
+
 template <uint32_t kWin> struct WaveLds {
   uint8_t sbuf[kWin + 288];  // window + slack for the 256-byte VGPR mirror
 };
@@ -63,188 +256,149 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   const uint32_t lane = lane_id();
   const uint64_t boff = blk_off[chunk];
   const uint64_t len = col_len[chunk];
-  if (boff > comp_bytes || len > comp_bytes - boff) {
+  if (!chunk_ok(boff, len, comp_bytes)) {
     if (lane == 0 && status) status[chunk] = kCorrupt;
     return;
   }
-  const uint8_t *src = comp + boff;
-  const uint32_t cap = (uint32_t)(chunk_bytes / w);
+  Walk s;
+  s.src = comp + boff;
+  s.len = (uint32_t)len;
+  s.cap = (uint32_t)(chunk_bytes / w);
+  s.ofast = s.cap >= 10 ? s.cap - 9 : 0;
+  s.o = 0;
   const uint64_t mask = width_mask(w);
   const uint32_t nbits = 8 * w;
   uint8_t *dst = out + chunk * chunk_bytes;
-  const uint32_t *sdw = reinterpret_cast<const uint32_t *>(L.sbuf);
 
-  // Stream offsets are 32-bit here: a chunk's stream is far below 4 GiB.
-  uint32_t wpos = 0;  // stream offset held at sbuf[lead]
-  uint32_t lead = 0, P = 0;
-  bool at_end = true;
-  // The walk stops at p >= pdone (stream consumed: P when the window holds the
-  // stream's end) and restages at p > pstop (header bytes might lie past the
-  // window: P - 32 when it does not).
-  uint32_t pdone = 0, pstop = 0;
-  // SHORT_REPEAT fast path bounds (a block is at most 10 bytes and 10 values):
-  // p < pfast keeps the header and the whole block inside the staged stream,
-  // o < ofast keeps the values inside the chunk.
-  uint32_t pfast = 0;
-  const uint32_t ofast = cap >= 10 ? cap - 9 : 0;
-  auto stage = [&](uint32_t pos) {
-    __syncthreads();  // earlier LDS reads of this wave are done before overwrite
-    const uint32_t n = ((uint32_t)len - pos) < (kWin - 16) ? ((uint32_t)len - pos) : (kWin - 16);
-    lead = 0;
-    if (n) stage_to_lds(L.sbuf, src, pos, n, lane, kWave, &lead);
-    P = lead + n;
-    at_end = (pos + n == (uint32_t)len);
-    wpos = pos;
-    pdone = at_end ? P : 0xffffffffu;
-    pstop = at_end ? 0xffffffffu : P - 32;
-    pfast = at_end ? (P >= 10 ? P - 9 : 0) : P - 31;
-    __syncthreads();
-  };
-
-  if (len >= 0xffffffffull) {  // outside the 32-bit stream offsets used below
-    if (lane == 0 && status) status[chunk] = kCorrupt;
-    return;
-  }
-  stage(0);
-  uint32_t p = lead, o = 0;
-  uint32_t vbase = p & ~3u;
-  uint32_t vw = sdw[vbase / 4 + lane];
+  walk_stage<kWin>(s, L.sbuf, 0, lane);
+  __syncthreads();
+  walk_mirror(s, L.sbuf, lane);
   int32_t st = kOk;
   uint32_t bad = 0;
   bool done = (len == 0);
   while (!done && st == kOk) {
-    // ---- walk: collect up to 64 blocks
-    uint32_t nd = 0, dpos = 0, dout = 0;
+    uint32_t dpos = 0, dout = 0;
     bool restage = false;
-    while (nd < (uint32_t)kWave) {
-      // Fast path for runs of SHORT_REPEAT blocks (the most frequent header):
-      // while the position and output index are far enough from the window
-      // end and the chunk end that no bound can be hit, a block costs one
-      // header read, two bit-field extracts and the lane record.
-      for (;;) {
-        if (nd >= (uint32_t)kWave || p >= pfast || o >= ofast) break;
-        if (p - vbase > 224) {
-          vbase = p & ~3u;
-          vw = sdw[vbase / 4 + lane];
-        }
-        const uint32_t k = (p - vbase) >> 2;
-        const uint32_t first = (uint32_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k) |
-                                           ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k + 1)
-                                            << 32)) >> ((p & 3) * 8)) & 0xff;
-        if (first >= 64) break;
-        if (lane == nd) {
-          dpos = p;
-          dout = o;
-        }
-        nd++;
-        o += (first & 7) + 3;
-        p += 2 + ((first >> 3) & 7);
-      }
-      if (nd >= (uint32_t)kWave) break;
-      if (p >= pdone) {  // decoder_new.h:1000 -- stream consumed
-        done = true;
-        break;
-      }
-      if (p > pstop) {  // header bytes might lie past the window
-        restage = true;
-        break;
-      }
-      if (p - vbase > 224) {
-        vbase = p & ~3u;
-        vw = sdw[vbase / 4 + lane];
-      }
-      // four header bytes from two adjacent dwords of the VGPR mirror
-      const uint32_t k = (p - vbase) >> 2;
-      const uint64_t pair = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k) |
-                            ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k + 1) << 32);
-      const uint32_t x = (uint32_t)(pair >> ((p & 3) * 8));
-      const uint32_t first = x & 0xff;
-      uint32_t size, cnt;
-      if (first < 64) {  // SHORT_REPEAT
-        size = 2 + ((first >> 3) & 7);
-        cnt = (first & 7) + 3;
-      } else {
-        const uint32_t kind = first >> 6;
-        const uint32_t code = (first >> 1) & 0x1f;
-        const uint32_t fbw = decode_width(code);
-        const uint32_t len8 = ((((first & 1) << 8) | (uint32_t)((x >> 8) & 0xff)) + 1) & 0xff;
-        if (kind == 1) {  // DIRECT
-          size = 2 + (len8 * fbw + 7) / 8;
-          cnt = len8;
-        } else if (kind == 3) {  // DELTA: two varints then packed deltas
-          const uint32_t nd8 = (len8 - 2) & 0xff;
-          uint32_t q = p + 2;
-          const uint32_t l1 = varint_len(L.sbuf, vw, vbase, q, P);
-          const uint32_t l2 = q + l1 < P ? varint_len(L.sbuf, vw, vbase, q + l1, P) : 1;
-          if (q + l1 + l2 <= P && (l1 > 10 || l2 > 10)) {  // longer than any 64-bit varint
-            st = kCorrupt;
-            break;
-          }
-          q += l1 + l2;
-          size = (q - p) + (code ? (nd8 * fbw + 7) / 8 : 0);
-          cnt = 2 + nd8;
-        } else {  // PATCHED_BASE
-          const uint32_t third = (uint32_t)((x >> 16) & 0xff), fourth = (uint32_t)((x >> 24) & 0xff);
-          const uint32_t pw = decode_width(third & 0x1f);
-          const uint32_t pgw = (fourth >> 5) + 1;
-          if (pw + pgw > 64) {  // beyond the 65-entry closest-width table
-            st = kCorrupt;
-            break;
-          }
-          size = 4 + (third >> 5) + 1 + (len8 * fbw + 7) / 8 +
-                 ((fourth & 0x1f) * closest_fixed(pw + pgw) + 7) / 8;
-          cnt = len8;
-        }
-      }
-      if (p + size > P) {
-        if (at_end) st = kTrunc;
-        else restage = true;
-        break;
-      }
-      if (o + cnt > cap) {
-        st = kOverflow;
-        break;
-      }
-      if (lane == nd) {
-        dpos = p;
-        dout = o;
-      }
-      nd++;
-      o += cnt;
-      p += size;
-    }
-    // ---- expand the collected blocks
-    const uint32_t myfirst = lane < nd ? L.sbuf[dpos] : 0u;
-    const bool is_sr = lane < nd && (myfirst >> 6) == 0;
-    if (is_sr) {  // SHORT_REPEAT: big-endian value, 3..10 copies (decoder_new.h:1003-1015)
-      const uint32_t nb = ((myfirst >> 3) & 7) + 1;
-      const uint32_t c = (myfirst & 7) + 3;
-      const uint64_t v = lds_be(L.sbuf + dpos + 1, nb) & mask;
-      for (uint32_t k = 0; k < c; k++) store_elem(dst + (uint64_t)(dout + k) * w, w, v);
-    }
-    uint64_t big = __ballot(lane < nd && !is_sr);
-    while (big) {
-      const uint32_t j = (uint32_t)__builtin_ctzll(big);
-      big &= big - 1;
-      const uint32_t bp = (uint32_t)__builtin_amdgcn_readlane(dpos, j);
-      const uint32_t bo = (uint32_t)__builtin_amdgcn_readlane(dout, j);
-      expand_big(L.sbuf, bp, bo, dst, w, mask, nbits, lane, &bad);
-    }
+    const uint32_t p0 = s.p;
+    const uint32_t nd = walk_batch(s, L.sbuf, lane, dpos, dout, done, restage, st);
+    expand_batch(L.sbuf, nd, ~0ull, dpos, dout, dst, w, mask, nbits, lane, &bad);
     // A block never exceeds ~2.3 KB, so a fresh window always makes progress;
     // no progress would mean a malformed stream.
-    if (restage && nd == 0 && p == lead) st = kCorrupt;
+    if (restage && nd == 0 && p0 == s.lead) st = kCorrupt;
     if (restage && st == kOk) {
-      stage(wpos + (p - lead));
-      p = lead;
-      vbase = p & ~3u;
-      vw = sdw[vbase / 4 + lane];
+      __syncthreads();  // this wave's LDS reads are done before the overwrite
+      walk_stage<kWin>(s, L.sbuf, s.wpos + (s.p - s.lead), lane);
+      __syncthreads();
+      walk_mirror(s, L.sbuf, lane);
     }
   }
   if (__ballot(bad) && st == kOk) st = kCorrupt;
   // positions the stream did not produce are zero (decoder_trans.h:81)
   if (st == kOk)
-    for (uint32_t i = o + lane; i < cap; i += kWave) store_elem(dst + (uint64_t)i * w, w, 0);
+    for (uint32_t i = s.o + lane; i < s.cap; i += kWave) store_elem(dst + (uint64_t)i * w, w, 0);
   if (lane == 0 && status) status[chunk] = st;
+}
+
+// ---------------------------------------------------------------- walker + expanders
+
+constexpr uint32_t kBatchLast = 1u << 30;     // no batch follows this one
+constexpr uint32_t kBatchRestage = 1u << 29;  // the window is restaged after this batch
+
+template <uint32_t kWin> struct PcLds {
+  uint8_t sbuf[kWin + 288];
+  uint32_t dpos[2][kWave];
+  uint32_t dout[2][kWave];
+  uint32_t info[2];  // nd | kBatchLast | kBatchRestage
+  uint32_t bad;
+};
+
+// Barrier protocol (every wave executes the same sequence): for batch k the
+// walker fills buffer k&1 and all waves meet at barrier B_k; the expanders
+// then write batch k while the walker walks batch k+1. A batch that ends
+// with a restage adds barrier R_k after which the walker (alone) reloads the
+// window; batch k+1 is published at B_{k+1}. Buffer k&1 is rewritten only
+// after B_{k+1}, by which time the expanders have finished batch k-1... the
+// two buffers alternate, so the walker never overwrites a batch in use.
+template <uint32_t kWin, uint32_t kExp>
+__global__ void __launch_bounds__(kWave *(1 + kExp))
+rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
+                const uint64_t *__restrict__ blk_off, const uint64_t *__restrict__ col_len,
+                uint64_t n_chunks, uint32_t w, uint64_t chunk_bytes, uint8_t *__restrict__ out,
+                int32_t *__restrict__ status) {
+  __shared__ PcLds<kWin> L;
+  const uint64_t chunk = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t boff = blk_off[chunk];
+  const uint64_t len = col_len[chunk];
+  if (!chunk_ok(boff, len, comp_bytes)) {  // uniform over the workgroup
+    if (threadIdx.x == 0 && status) status[chunk] = kCorrupt;
+    return;
+  }
+  const uint32_t cap = (uint32_t)(chunk_bytes / w);
+  const uint64_t mask = width_mask(w);
+  const uint32_t nbits = 8 * w;
+  uint8_t *dst = out + chunk * chunk_bytes;
+  if (threadIdx.x == 0) L.bad = 0;
+
+  if (wave == 0) {  // ---- walker
+    Walk s;
+    s.src = comp + boff;
+    s.len = (uint32_t)len;
+    s.cap = cap;
+    s.ofast = cap >= 10 ? cap - 9 : 0;
+    s.o = 0;
+    walk_stage<kWin>(s, L.sbuf, 0, lane);
+    walk_mirror(s, L.sbuf, lane);  // the walker's own LDS writes precede its reads
+    int32_t st = kOk;
+    bool done = (len == 0);
+    for (uint32_t k = 0;; k++) {
+      uint32_t dpos = 0, dout = 0, nd = 0;
+      bool restage = false;
+      if (!done && st == kOk) {
+        const uint32_t p0 = s.p;
+        nd = walk_batch(s, L.sbuf, lane, dpos, dout, done, restage, st);
+        if (restage && nd == 0 && p0 == s.lead) st = kCorrupt;
+      }
+      const bool last = done || st != kOk;
+      if (last) restage = false;
+      L.dpos[k & 1][lane] = dpos;
+      L.dout[k & 1][lane] = dout;
+      if (lane == 0) L.info[k & 1] = nd | (last ? kBatchLast : 0u) | (restage ? kBatchRestage : 0u);
+      __syncthreads();  // B_k
+      if (last) {
+        __syncthreads();  // F: expanders are done
+        const uint32_t bad = L.bad;
+        if (bad && st == kOk) st = kCorrupt;
+        if (st == kOk)
+          for (uint32_t i = s.o + lane; i < cap; i += kWave) store_elem(dst + (uint64_t)i * w, w, 0);
+        if (lane == 0 && status) status[chunk] = st;
+        return;
+      }
+      if (restage) {
+        __syncthreads();  // R_k: the expanders have finished batch k
+        walk_stage<kWin>(s, L.sbuf, s.wpos + (s.p - s.lead), lane);
+        walk_mirror(s, L.sbuf, lane);
+      }
+    }
+  } else {  // ---- expanders: wave e takes blocks j with j % kExp == e - 1
+    uint64_t sel = 0;
+    for (uint32_t j = wave - 1; j < (uint32_t)kWave; j += kExp) sel |= 1ull << j;
+    uint32_t bad = 0;
+    for (uint32_t k = 0;; k++) {
+      __syncthreads();  // B_k
+      const uint32_t info = L.info[k & 1];
+      const uint32_t nd = info & 0xffff;
+      expand_batch(L.sbuf, nd, sel, L.dpos[k & 1][lane], L.dout[k & 1][lane], dst, w, mask, nbits,
+                   lane, &bad);
+      if (info & kBatchLast) {
+        if (__ballot(bad)) L.bad = 1;
+        __syncthreads();  // F
+        return;
+      }
+      if (info & kBatchRestage) __syncthreads();  // R_k
+    }
+  }
 }
 
 }  // namespace rle2
@@ -262,6 +416,24 @@ hipError_t launch_rlev2_wave(const uint8_t *comp, uint64_t comp_bytes, const uin
     hipLaunchKernelGGL(rle2::rlev2_wave_kernel<8192>, dim3((uint32_t)n_chunks), dim3(kWaveSize), 0,
                        stream, comp, comp_bytes, blk_off, col_len, n_chunks, w, chunk_bytes, out,
                        status);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_rlev2_pc(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
+                           const uint64_t *col_len, uint64_t n_chunks, uint32_t w,
+                           uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t nexp,
+                           hipStream_t stream) {
+  if (n_chunks == 0) return hipSuccess;
+  if (nexp == 1)
+    hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, 1>), dim3((uint32_t)n_chunks),
+                       dim3(2 * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len, n_chunks,
+                       w, chunk_bytes, out, status);
+  else if (nexp == 3)
+    hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, 3>), dim3((uint32_t)n_chunks),
+                       dim3(4 * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len, n_chunks,
+                       w, chunk_bytes, out, status);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -262,16 +262,45 @@ def test_v2_error_status_matches_oracle(ctx, k):
     assert rc != 0 and st.tolist() == [rc]
 
 
-@pytest.mark.parametrize("shape", [(64, 4096), (64, 8192), (256, 4096), (128, 4096), (512, 8192)])
+@pytest.mark.parametrize("shape", [(64, 4096, 0), (64, 8192, 0), (256, 4096, 0), (128, 4096, 0),
+                                   (512, 8192, 0), (128, 4096, 2), (256, 4096, 2)])
 def test_v2_every_launch_shape(ctx, shape):
     say(f"rlev2 launch shape {shape}")
     raw, comp, blk, cl, n = ca.synth_rlev2(12 * CH // 8 + 5, 8, CH, seed=77)
-    ctx.tune_set(ca.RLEV2, ca.TuneParams(shape[0], shape[1], 0, 0))
+    saved = ctx.tune_get(ca.RLEV2)
+    ctx.tune_set(ca.RLEV2, ca.TuneParams(shape[0], shape[1], 0, shape[2]))
     try:
         st, out = _check_chunks(ctx, ca.RLEV2, comp, blk, cl, 8, CH)
         assert np.all(st == 0) and out[: len(raw)].tobytes() == raw.tobytes()
     finally:
-        ctx.tune_set(ca.RLEV2, ca.TuneParams(64, 4096, 0, 0))
+        ctx.tune_set(ca.RLEV2, saved)
+
+
+@pytest.mark.parametrize("shape", [(128, 4096, 2), (256, 4096, 2), (64, 4096, 0)])
+def test_v2_walker_and_expander_kernels(ctx, shape):
+    # the one-wave kernel and the walker + expander kernels on the same cases:
+    # synthetic columns of several widths (many windows and batches per
+    # chunk), the malformed streams, an empty stream and random PATCHED_BASE
+    say(f"rlev2 kernel shape {shape}")
+    saved = ctx.tune_get(ca.RLEV2)
+    ctx.tune_set(ca.RLEV2, ca.TuneParams(shape[0], shape[1], 0, shape[2]))
+    try:
+        for width in (8, 4, 1):
+            raw, comp, blk, cl, n = ca.synth_rlev2(24 * CH // width + 77, width, CH,
+                                                   seed=121 + width)
+            st, out = _check_chunks(ctx, ca.RLEV2, comp, blk, cl, width, CH)
+            assert np.all(st == 0) and out[: len(raw)].tobytes() == raw.tobytes()
+        for hexs, cap in V2_BAD + [("", 16)]:
+            s = bytes.fromhex(hexs)
+            comp, blk, cl = single_chunk(s, pad=16)
+            st, out = rle_decode(ctx, ca.RLEV2, comp, blk, cl, 8, cap * 8)
+            rc, ref = oracle.rlev2_decode_chunk(s, 8, cap)
+            assert st.tolist() == [rc], hexs
+            if rc == 0:
+                assert out.view(np.uint64).tolist() == ref.tolist() + [0] * (cap - len(ref))
+        test_v2_patched_base_random(ctx)
+    finally:
+        ctx.tune_set(ca.RLEV2, saved)
 
 
 def test_v2_patch_outside_block_is_corrupt(ctx):
