@@ -76,16 +76,18 @@ codag_status check_rle_args(uint32_t w, uint64_t chunk_bytes) {
   return CODAG_OK;
 }
 
-// RLE v2 walker + expander launch: 2 or 4 waves per chunk, 4 KiB window,
+// RLE v1 / v2 walker + expander launch: 2 or 4 waves per chunk, 4 KiB window,
 // queue_depth 2 (the double-buffered descriptor queue).
-bool rlev2_pc_shape(const codag_tune_params &t) {
+bool pc_shape(const codag_tune_params &t) {
   return t.queue_depth == 2 && t.window_bytes == 4096 &&
          (t.threads_per_group == 128 || t.threads_per_group == 256);
 }
 
 void default_tune(codag_ctx *c) {
-  c->tune[CODAG_RLEV1] = {256, 4096, 0, 0};
-  c->tune[CODAG_RLEV2] = {64, 4096, 0, 0};
+  // defaults: the fastest shapes on the benchmark columns (MI355X); the
+  // autotuner can pick per column
+  c->tune[CODAG_RLEV1] = {512, 4096, 0, 0};
+  c->tune[CODAG_RLEV2] = {256, 4096, 0, 2};
   // 28 waves per CU: the inflate kernel's LDS and register budget allow seven per SIMD
   c->tune[CODAG_DEFLATE] = {64, 0, 28, 1};
   c->tune[0] = {0, 0, 0, 0};
@@ -195,9 +197,13 @@ codag_status codag_rlev1_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t 
   if (n_chunks > 0x7fffffffull) return fail(CODAG_ERR_INVALID, "rlev1_decode: too many chunks");
   DeviceGuard g(ctx->device);
   const codag_tune_params &t = ctx->tune[CODAG_RLEV1];
-  HIP_TRY(launch_rlev1_decode(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
-                              d_out, d_status, t.threads_per_group, t.window_bytes,
-                              pick(ctx, stream)));
+  if (pc_shape(t))  // walker wave + expander waves per chunk
+    HIP_TRY(launch_rlev1_pc(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
+                            d_out, d_status, t.threads_per_group / 64 - 1, pick(ctx, stream)));
+  else
+    HIP_TRY(launch_rlev1_decode(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w,
+                                chunk_bytes, d_out, d_status, t.threads_per_group, t.window_bytes,
+                                pick(ctx, stream)));
   return CODAG_OK;
 }
 
@@ -212,7 +218,7 @@ codag_status codag_rlev2_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t 
   if (n_chunks > 0x7fffffffull) return fail(CODAG_ERR_INVALID, "rlev2_decode: too many chunks");
   DeviceGuard g(ctx->device);
   const codag_tune_params &t = ctx->tune[CODAG_RLEV2];
-  if (rlev2_pc_shape(t))  // walker wave + expander waves per chunk
+  if (pc_shape(t))  // walker wave + expander waves per chunk
     HIP_TRY(launch_rlev2_pc(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
                             d_out, d_status, t.threads_per_group / 64 - 1, pick(ctx, stream)));
   else if (t.threads_per_group == 64)  // one self-contained wave per chunk
@@ -450,10 +456,10 @@ codag_status codag_tune_get(codag_ctx *ctx, codag_codec codec, codag_tune_params
 codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_params *p) {
   if (!ctx || !p || codec < CODAG_RLEV1 || codec > CODAG_DEFLATE)
     return fail(CODAG_ERR_INVALID, "tune_set: bad argument");
-  const bool wave_shape = codec == CODAG_RLEV2 &&
-                          ((p->threads_per_group == 64 && p->queue_depth == 0 &&
-                            (p->window_bytes == 4096 || p->window_bytes == 8192)) ||
-                           rlev2_pc_shape(*p));
+  const bool wave_shape = (codec == CODAG_RLEV2 && p->threads_per_group == 64 &&
+                           p->queue_depth == 0 &&
+                           (p->window_bytes == 4096 || p->window_bytes == 8192)) ||
+                          (codec != CODAG_DEFLATE && pc_shape(*p));
   if (codec != CODAG_DEFLATE && !wave_shape &&
       !rle_shape_supported(p->threads_per_group, p->window_bytes))
     return fail(CODAG_ERR_INVALID, "tune_set: launch shape not compiled in");
@@ -476,9 +482,9 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
   if (codec == CODAG_DEFLATE) {
     for (uint32_t per : {8u, 16u, 20u, 24u, 28u}) cands.push_back({64, 0, per, 1});
   } else {
-    if (codec == CODAG_RLEV2) {  // walker/expander and one-wave-per-chunk shapes first
-      cands.push_back({256, 4096, 0, 2});
-      cands.push_back({128, 4096, 0, 2});
+    cands.push_back({256, 4096, 0, 2});  // walker + expander shapes first
+    cands.push_back({128, 4096, 0, 2});
+    if (codec == CODAG_RLEV2) {  // one-wave-per-chunk shapes
       cands.push_back({64, 4096, 0, 0});
       cands.push_back({64, 8192, 0, 0});
     }

@@ -11,6 +11,10 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Marks a wave-uniform value as such (v_readfirstlane), so code that depends
+// on it stays in scalar registers with scalar branches.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ uint64_t width_mask(uint32_t w) {
   return w >= 8 ? ~0ull : ((1ull << (8 * w)) - 1);
 }

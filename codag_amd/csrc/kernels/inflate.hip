@@ -115,7 +115,6 @@ struct Lds {
   uint8_t lens[320];
 };
 
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t j) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
 }

@@ -105,7 +105,16 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
                                                bool &restage, int32_t &st) {
   const uint32_t *sdw = reinterpret_cast<const uint32_t *>(sbuf);
   uint32_t nd = 0;
-  uint32_t p = s.p, o = s.o, vbase = s.vbase, vw = s.vw;
+  // The walk state is wave-uniform; saying so keeps the loop in scalar
+  // registers with scalar branches (the compiler cannot prove it through the
+  // window bookkeeping).
+  uint32_t p = uni(s.p), o = uni(s.o), vbase = uni(s.vbase), vw = s.vw;
+  s.P = uni(s.P);
+  s.cap = uni(s.cap);
+  s.ofast = uni(s.ofast);
+  s.pdone = uni(s.pdone);
+  s.pstop = uni(s.pstop);
+  s.pfast = uni(s.pfast);
   while (nd < (uint32_t)kWave) {
     // Fast path for runs of SHORT_REPEAT blocks (the most frequent header):
     // while the position and output index are far enough from the window

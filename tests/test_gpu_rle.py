@@ -128,6 +128,48 @@ def test_v1_error_status_matches_oracle(ctx, k):
     assert rc != 0 and st.tolist() == [rc]
 
 
+@pytest.mark.parametrize("shape", [(256, 4096, 2), (128, 4096, 2)])
+def test_v1_walker_and_expander_kernels(ctx, shape):
+    # the walker + expander kernel on the synthetic columns, the known answers,
+    # the malformed corpus, small chunks, and randomly damaged streams whose
+    # status (first error in stream order) must equal the oracle's
+    say(f"rlev1 kernel shape {shape}")
+    saved = ctx.tune_get(ca.RLEV1)
+    ctx.tune_set(ca.RLEV1, ca.TuneParams(shape[0], shape[1], 0, shape[2]))
+    try:
+        for width in (8, 4, 1):
+            raw, cont, blk, cl, n = ca.synth_rlev1(24 * CH // width + 333, width, CH, 1 << 16,
+                                                   seed=31 + width)
+            st, out = _check_chunks(ctx, ca.RLEV1, cont[8:], blk, cl, width, CH)
+            assert np.all(st == 0) and out[: len(raw)].tobytes() == raw.tobytes()
+        for k in range(len(V1_KATS)):
+            test_v1_kats(ctx, k)
+        for k in range(len(V1_BAD)):
+            test_v1_error_status_matches_oracle(ctx, k)
+        test_v1_many_small_chunks_and_max_segments(ctx)
+        rng = np.random.default_rng(17)
+        base = []
+        for b in range(6):
+            v = [int(x) for x in rng.integers(0, 1 << (8 * (b + 1)), 200)]
+            v[50:90] = [v[50]] * 40
+            base.append(pc.rlev1_encode(v, 8))
+        streams = []
+        for t in range(600):
+            s = bytearray(base[t % 6])
+            for _ in range(int(rng.integers(1, 4))):
+                s[int(rng.integers(0, len(s)))] = int(rng.integers(0, 256))
+            if t % 3 == 0:
+                s = s[: int(rng.integers(0, len(s) + 1))]
+            streams.append(bytes(s))
+        off = np.cumsum([0] + [len(s) for s in streams])[:-1].astype(np.uint64)
+        cl = np.array([len(s) for s in streams], np.uint64)
+        comp = np.frombuffer(b"".join(streams) + b"\0" * 16, np.uint8)
+        st, _ = _check_chunks(ctx, ca.RLEV1, comp, off, cl, 8, 200 * 8)
+        assert len(set(st.tolist())) > 2  # the damage produced several kinds of status
+    finally:
+        ctx.tune_set(ca.RLEV1, saved)
+
+
 def test_v1_stream_outside_buffer_is_corrupt(ctx):
     say("rlev1 out-of-range blk_off")
     comp = np.zeros(64, np.uint8)
