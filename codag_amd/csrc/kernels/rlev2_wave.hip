@@ -64,13 +64,15 @@ struct Walk {
   uint32_t ofast;      // o < ofast keeps a SHORT_REPEAT block inside the chunk
   uint32_t wpos;       // stream offset held at sbuf[lead]
   uint32_t lead, P;    // window: bytes [lead, P) of sbuf hold the stream
-  bool at_end;         // the window holds the stream's end
+  uint32_t at_end;     // the window holds the stream's end
   // The walk stops at p >= pdone (stream consumed: P when the window holds
   // the stream's end), restages at p > pstop (header bytes might lie past
   // the window: P - 32 when it does not), and takes the SHORT_REPEAT fast
   // path while p < pfast (a block is at most 10 bytes).
   uint32_t pdone, pstop, pfast;
   uint32_t p, o;       // walk position (LDS offset) and values produced
+  uint32_t stop;       // outcome of the last walk_batch (kStop*)
+  int32_t err;         // error status of the last walk_batch
   uint32_t vbase, vw;  // VGPR mirror of LDS bytes [vbase, vbase + 256)
 };
 
@@ -97,18 +99,23 @@ __device__ __forceinline__ void walk_mirror(Walk &s, const uint8_t *sbuf, uint32
   s.vw = reinterpret_cast<const uint32_t *>(sbuf)[s.vbase / 4 + lane];
 }
 
+enum : uint32_t { kStopNone = 0, kStopDone = 1, kStopRestage = 2 };
+
 // Collects up to 64 blocks: lane j receives block j's LDS offset (dpos) and
-// first value index (dout). Sets *done when the stream is consumed, *restage
-// when the next header or block may lie past the window, *st on an error.
+// first value index (dout). Sets s.stop to kStopDone when the stream is
+// consumed or kStopRestage when the next header or block may lie past the
+// window, and s.err on an error. (Outcomes travel as integers in the walk
+// state: flags passed by reference would be kept in scratch memory.)
 __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uint32_t lane,
-                                               uint32_t &dpos, uint32_t &dout, bool &done,
-                                               bool &restage, int32_t &st) {
+                                               uint32_t &dpos, uint32_t &dout) {
   const uint32_t *sdw = reinterpret_cast<const uint32_t *>(sbuf);
   uint32_t nd = 0;
   // The walk state is wave-uniform; saying so keeps the loop in scalar
   // registers with scalar branches (the compiler cannot prove it through the
   // window bookkeeping).
   uint32_t p = uni(s.p), o = uni(s.o), vbase = uni(s.vbase), vw = s.vw;
+  uint32_t stop = kStopNone;
+  int32_t err = kOk;
   s.P = uni(s.P);
   s.cap = uni(s.cap);
   s.ofast = uni(s.ofast);
@@ -141,11 +148,11 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
     }
     if (nd >= (uint32_t)kWave) break;
     if (p >= s.pdone) {  // decoder_new.h:1000 -- stream consumed
-      done = true;
+      stop = kStopDone;
       break;
     }
     if (p > s.pstop) {  // header bytes might lie past the window
-      restage = true;
+      stop = kStopRestage;
       break;
     }
     if (p - vbase > 224) {
@@ -176,7 +183,7 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
         const uint32_t l1 = varint_len(sbuf, vw, vbase, q, s.P);
         const uint32_t l2 = q + l1 < s.P ? varint_len(sbuf, vw, vbase, q + l1, s.P) : 1;
         if (q + l1 + l2 <= s.P && (l1 > 10 || l2 > 10)) {  // longer than any 64-bit varint
-          st = kCorrupt;
+          err = kCorrupt;
           break;
         }
         q += l1 + l2;
@@ -187,7 +194,7 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
         const uint32_t pw = decode_width(third & 0x1f);
         const uint32_t pgw = (fourth >> 5) + 1;
         if (pw + pgw > 64) {  // beyond the 65-entry closest-width table
-          st = kCorrupt;
+          err = kCorrupt;
           break;
         }
         size = 4 + (third >> 5) + 1 + (len8 * fbw + 7) / 8 +
@@ -196,12 +203,12 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
       }
     }
     if (p + size > s.P) {
-      if (s.at_end) st = kTrunc;
-      else restage = true;
+      if (s.at_end) err = kTrunc;
+      else stop = kStopRestage;
       break;
     }
     if (o + cnt > s.cap) {
-      st = kOverflow;
+      err = kOverflow;
       break;
     }
     if (lane == nd) {
@@ -212,6 +219,8 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
     o += cnt;
     p += size;
   }
+  s.stop = stop;
+  s.err = err;
   s.p = p;
   s.o = o;
   s.vbase = vbase;
@@ -220,19 +229,20 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
 }
 
 // Expands the blocks of a batch (lane j holds block j's dpos/dout, j < nd)
-// that `sel` selects. All 64 lanes of the wave must call it.
+// that `sel` selects, for a column of W-byte elements. All 64 lanes of the
+// wave must call it.
+template <uint32_t W>
 __device__ __forceinline__ void expand_batch(const uint8_t *sbuf, uint32_t nd, uint64_t sel,
-                                             uint32_t dpos, uint32_t dout, uint8_t *dst, uint32_t w,
-                                             uint64_t mask, uint32_t nbits, uint32_t lane,
-                                             uint32_t *bad) {
+                                             uint32_t dpos, uint32_t dout, uint8_t *dst,
+                                             uint32_t lane, uint32_t *bad) {
   const bool mine = lane < nd && ((sel >> lane) & 1);
   const uint32_t myfirst = mine ? sbuf[dpos] : 0u;
   const bool is_sr = mine && (myfirst >> 6) == 0;
   if (is_sr) {  // SHORT_REPEAT: big-endian value, 3..10 copies (decoder_new.h:1003-1015)
     const uint32_t nb = ((myfirst >> 3) & 7) + 1;
     const uint32_t c = (myfirst & 7) + 3;
-    const uint64_t v = lds_be(sbuf + dpos + 1, nb) & mask;
-    for (uint32_t k = 0; k < c; k++) store_elem(dst + (uint64_t)(dout + k) * w, w, v);
+    const auto v = be_bytes<W>(sbuf, dpos + 1, nb);
+    for (uint32_t k = 0; k < c; k++) store_w<W>(dst, dout + k, v);
   }
   uint64_t big = __ballot(mine && !is_sr);
   while (big) {
@@ -240,7 +250,7 @@ __device__ __forceinline__ void expand_batch(const uint8_t *sbuf, uint32_t nd, u
     big &= big - 1;
     const uint32_t bp = (uint32_t)__builtin_amdgcn_readlane(dpos, j);
     const uint32_t bo = (uint32_t)__builtin_amdgcn_readlane(dout, j);
-    expand_big(sbuf, bp, bo, dst, w, mask, nbits, lane, bad);
+    expand_big<W>(sbuf, bp, bo, dst, lane, bad);
   }
 }
 
@@ -254,11 +264,11 @@ template <uint32_t kWin> struct WaveLds {
   uint8_t sbuf[kWin + 288];  // window + slack for the 256-byte VGPR mirror
 };
 
-template <uint32_t kWin>
+template <uint32_t kWin, uint32_t W>
 __global__ void __launch_bounds__(kWave)
 rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                   const uint64_t *__restrict__ blk_off, const uint64_t *__restrict__ col_len,
-                  uint64_t n_chunks, uint32_t w, uint64_t chunk_bytes, uint8_t *__restrict__ out,
+                  uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *__restrict__ out,
                   int32_t *__restrict__ status) {
   __shared__ WaveLds<kWin> L;
   const uint64_t chunk = blockIdx.x;
@@ -272,11 +282,9 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   Walk s;
   s.src = comp + boff;
   s.len = (uint32_t)len;
-  s.cap = (uint32_t)(chunk_bytes / w);
+  s.cap = (uint32_t)(chunk_bytes / W);
   s.ofast = s.cap >= 10 ? s.cap - 9 : 0;
   s.o = 0;
-  const uint64_t mask = width_mask(w);
-  const uint32_t nbits = 8 * w;
   uint8_t *dst = out + chunk * chunk_bytes;
 
   walk_stage<kWin>(s, L.sbuf, 0, lane);
@@ -287,10 +295,12 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   bool done = (len == 0);
   while (!done && st == kOk) {
     uint32_t dpos = 0, dout = 0;
-    bool restage = false;
     const uint32_t p0 = s.p;
-    const uint32_t nd = walk_batch(s, L.sbuf, lane, dpos, dout, done, restage, st);
-    expand_batch(L.sbuf, nd, ~0ull, dpos, dout, dst, w, mask, nbits, lane, &bad);
+    const uint32_t nd = walk_batch(s, L.sbuf, lane, dpos, dout);
+    done = s.stop == kStopDone;
+    const bool restage = s.stop == kStopRestage;
+    st = s.err;
+    expand_batch<W>(L.sbuf, nd, ~0ull, dpos, dout, dst, lane, &bad);
     // A block never exceeds ~2.3 KB, so a fresh window always makes progress;
     // no progress would mean a malformed stream.
     if (restage && nd == 0 && p0 == s.lead) st = kCorrupt;
@@ -304,7 +314,7 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   if (__ballot(bad) && st == kOk) st = kCorrupt;
   // positions the stream did not produce are zero (decoder_trans.h:81)
   if (st == kOk)
-    for (uint32_t i = s.o + lane; i < s.cap; i += kWave) store_elem(dst + (uint64_t)i * w, w, 0);
+    for (uint32_t i = s.o + lane; i < s.cap; i += kWave) store_w<W>(dst, i, 0);
   if (lane == 0 && status) status[chunk] = st;
 }
 
@@ -328,11 +338,11 @@ template <uint32_t kWin> struct PcLds {
 // window; batch k+1 is published at B_{k+1}. Buffer k&1 is rewritten only
 // after B_{k+1}, by which time the expanders have finished batch k-1... the
 // two buffers alternate, so the walker never overwrites a batch in use.
-template <uint32_t kWin, uint32_t kExp>
+template <uint32_t kWin, uint32_t kExp, uint32_t W>
 __global__ void __launch_bounds__(kWave *(1 + kExp))
 rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                 const uint64_t *__restrict__ blk_off, const uint64_t *__restrict__ col_len,
-                uint64_t n_chunks, uint32_t w, uint64_t chunk_bytes, uint8_t *__restrict__ out,
+                uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *__restrict__ out,
                 int32_t *__restrict__ status) {
   __shared__ PcLds<kWin> L;
   const uint64_t chunk = blockIdx.x;
@@ -344,9 +354,7 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
     if (threadIdx.x == 0 && status) status[chunk] = kCorrupt;
     return;
   }
-  const uint32_t cap = (uint32_t)(chunk_bytes / w);
-  const uint64_t mask = width_mask(w);
-  const uint32_t nbits = 8 * w;
+  const uint32_t cap = (uint32_t)(chunk_bytes / W);
   uint8_t *dst = out + chunk * chunk_bytes;
   if (threadIdx.x == 0) L.bad = 0;
 
@@ -366,7 +374,10 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
       bool restage = false;
       if (!done && st == kOk) {
         const uint32_t p0 = s.p;
-        nd = walk_batch(s, L.sbuf, lane, dpos, dout, done, restage, st);
+        nd = walk_batch(s, L.sbuf, lane, dpos, dout);
+        done = s.stop == kStopDone;
+        restage = s.stop == kStopRestage;
+        st = s.err;
         if (restage && nd == 0 && p0 == s.lead) st = kCorrupt;
       }
       const bool last = done || st != kOk;
@@ -380,7 +391,7 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
         const uint32_t bad = L.bad;
         if (bad && st == kOk) st = kCorrupt;
         if (st == kOk)
-          for (uint32_t i = s.o + lane; i < cap; i += kWave) store_elem(dst + (uint64_t)i * w, w, 0);
+          for (uint32_t i = s.o + lane; i < cap; i += kWave) store_w<W>(dst, i, 0);
         if (lane == 0 && status) status[chunk] = st;
         return;
       }
@@ -398,8 +409,7 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
       __syncthreads();  // B_k
       const uint32_t info = L.info[k & 1];
       const uint32_t nd = info & 0xffff;
-      expand_batch(L.sbuf, nd, sel, L.dpos[k & 1][lane], L.dout[k & 1][lane], dst, w, mask, nbits,
-                   lane, &bad);
+      expand_batch<W>(L.sbuf, nd, sel, L.dpos[k & 1][lane], L.dout[k & 1][lane], dst, lane, &bad);
       if (info & kBatchLast) {
         if (__ballot(bad)) L.bad = 1;
         __syncthreads();  // F
@@ -412,21 +422,52 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
 
 }  // namespace rle2
 
+namespace {
+
+template <uint32_t kWin, uint32_t W>
+void wave_launch(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
+                 const uint64_t *col_len, uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *out,
+                 int32_t *status, hipStream_t stream) {
+  hipLaunchKernelGGL((rle2::rlev2_wave_kernel<kWin, W>), dim3((uint32_t)n_chunks), dim3(kWaveSize),
+                     0, stream, comp, comp_bytes, blk_off, col_len, n_chunks, chunk_bytes, out,
+                     status);
+}
+
+template <uint32_t kExp, uint32_t W>
+void pc_launch(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
+               const uint64_t *col_len, uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *out,
+               int32_t *status, hipStream_t stream) {
+  hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, kExp, W>), dim3((uint32_t)n_chunks),
+                     dim3((1 + kExp) * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len,
+                     n_chunks, chunk_bytes, out, status);
+}
+
+// Instantiates F<..., W> for the element width w (1, 2, 4 or 8).
+#define CODAG_BY_WIDTH(F, ...)                     \
+  switch (w) {                                     \
+  case 1: F<__VA_ARGS__, 1>(ARGS); break;          \
+  case 2: F<__VA_ARGS__, 2>(ARGS); break;          \
+  case 4: F<__VA_ARGS__, 4>(ARGS); break;          \
+  case 8: F<__VA_ARGS__, 8>(ARGS); break;          \
+  default: return hipErrorInvalidValue;            \
+  }
+
+}  // namespace
+
 hipError_t launch_rlev2_wave(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
                              const uint64_t *col_len, uint64_t n_chunks, uint32_t w,
                              uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t win,
                              hipStream_t stream) {
   if (n_chunks == 0) return hipSuccess;
-  if (win == 4096)
-    hipLaunchKernelGGL(rle2::rlev2_wave_kernel<4096>, dim3((uint32_t)n_chunks), dim3(kWaveSize), 0,
-                       stream, comp, comp_bytes, blk_off, col_len, n_chunks, w, chunk_bytes, out,
-                       status);
-  else if (win == 8192)
-    hipLaunchKernelGGL(rle2::rlev2_wave_kernel<8192>, dim3((uint32_t)n_chunks), dim3(kWaveSize), 0,
-                       stream, comp, comp_bytes, blk_off, col_len, n_chunks, w, chunk_bytes, out,
-                       status);
-  else
+#define ARGS comp, comp_bytes, blk_off, col_len, n_chunks, chunk_bytes, out, status, stream
+  if (win == 4096) {
+    CODAG_BY_WIDTH(wave_launch, 4096)
+  } else if (win == 8192) {
+    CODAG_BY_WIDTH(wave_launch, 8192)
+  } else {
     return hipErrorInvalidValue;
+  }
+#undef ARGS
   return hipGetLastError();
 }
 
@@ -435,17 +476,18 @@ hipError_t launch_rlev2_pc(const uint8_t *comp, uint64_t comp_bytes, const uint6
                            uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t nexp,
                            hipStream_t stream) {
   if (n_chunks == 0) return hipSuccess;
-  if (nexp == 1)
-    hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, 1>), dim3((uint32_t)n_chunks),
-                       dim3(2 * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len, n_chunks,
-                       w, chunk_bytes, out, status);
-  else if (nexp == 3)
-    hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, 3>), dim3((uint32_t)n_chunks),
-                       dim3(4 * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len, n_chunks,
-                       w, chunk_bytes, out, status);
-  else
+#define ARGS comp, comp_bytes, blk_off, col_len, n_chunks, chunk_bytes, out, status, stream
+  if (nexp == 1) {
+    CODAG_BY_WIDTH(pc_launch, 1)
+  } else if (nexp == 3) {
+    CODAG_BY_WIDTH(pc_launch, 3)
+  } else {
     return hipErrorInvalidValue;
+  }
+#undef ARGS
   return hipGetLastError();
 }
+
+#undef CODAG_BY_WIDTH
 
 }  // namespace codag
