@@ -198,8 +198,14 @@ def bench_rle(ctx, args, cfg):
     gbs = out_bytes / (ms_step / 1e3) / 1e9
     k_ms = per[dom] / args.steps
     achieved = cols[dom].alg_bytes / (k_ms / 1e3) / 1e9
-    if cols[dom].codec == ca.RLEV1:
-        kname = "rlev1_decode_kernel"
+    if cols[dom].codec == ca.RLEV1:  # the launch shape selects the kernel
+        t = cols[dom].tune or ctx.tune_get(ca.RLEV1)
+        if t.queue_depth == 4 and t.window_bytes == 2048:
+            kname = "rlev1_pipe_kernel"
+        elif t.queue_depth == 2:
+            kname = "rlev1_pc_kernel"
+        else:
+            kname = "rlev1_decode_kernel"
     else:  # the launch shape selects the kernel
         t = cols[dom].tune or ctx.tune_get(ca.RLEV2)
         if t.queue_depth == 2 and t.threads_per_group in (128, 256):

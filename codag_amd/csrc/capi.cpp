@@ -83,10 +83,15 @@ bool pc_shape(const codag_tune_params &t) {
          (t.threads_per_group == 128 || t.threads_per_group == 256);
 }
 
+// RLE v1 pipeline launch: 8 waves per chunk, 2 KiB windows in a ring of 4.
+bool pipe_shape(const codag_tune_params &t) {
+  return t.threads_per_group == 512 && t.window_bytes == 2048 && t.queue_depth == 4;
+}
+
 void default_tune(codag_ctx *c) {
   // defaults: the fastest shapes on the benchmark columns (MI355X); the
   // autotuner can pick per column
-  c->tune[CODAG_RLEV1] = {512, 4096, 0, 0};
+  c->tune[CODAG_RLEV1] = {512, 2048, 0, 4};
   c->tune[CODAG_RLEV2] = {256, 4096, 0, 2};
   // 28 waves per CU: the inflate kernel's LDS and register budget allow seven per SIMD
   c->tune[CODAG_DEFLATE] = {64, 0, 28, 1};
@@ -197,7 +202,13 @@ codag_status codag_rlev1_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t 
   if (n_chunks > 0x7fffffffull) return fail(CODAG_ERR_INVALID, "rlev1_decode: too many chunks");
   DeviceGuard g(ctx->device);
   const codag_tune_params &t = ctx->tune[CODAG_RLEV1];
-  if (pc_shape(t))  // walker wave + expander waves per chunk
+  if (pipe_shape(t) && chunk_bytes / w <= (1u << 17))  // pipelined walker + helpers
+    HIP_TRY(launch_rlev1_pipe(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
+                              d_out, d_status, pick(ctx, stream)));
+  else if (pipe_shape(t))  // values per chunk beyond the pipeline's descriptor field
+    HIP_TRY(launch_rlev1_decode(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w,
+                                chunk_bytes, d_out, d_status, 512, 4096, pick(ctx, stream)));
+  else if (pc_shape(t))  // walker wave + expander waves per chunk
     HIP_TRY(launch_rlev1_pc(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
                             d_out, d_status, t.threads_per_group / 64 - 1, pick(ctx, stream)));
   else
@@ -459,7 +470,8 @@ codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_
   const bool wave_shape = (codec == CODAG_RLEV2 && p->threads_per_group == 64 &&
                            p->queue_depth == 0 &&
                            (p->window_bytes == 4096 || p->window_bytes == 8192)) ||
-                          (codec != CODAG_DEFLATE && pc_shape(*p));
+                          (codec != CODAG_DEFLATE && pc_shape(*p)) ||
+                          (codec == CODAG_RLEV1 && pipe_shape(*p));
   if (codec != CODAG_DEFLATE && !wave_shape &&
       !rle_shape_supported(p->threads_per_group, p->window_bytes))
     return fail(CODAG_ERR_INVALID, "tune_set: launch shape not compiled in");
@@ -482,7 +494,8 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
   if (codec == CODAG_DEFLATE) {
     for (uint32_t per : {8u, 16u, 20u, 24u, 28u}) cands.push_back({64, 0, per, 1});
   } else {
-    cands.push_back({256, 4096, 0, 2});  // walker + expander shapes first
+    if (codec == CODAG_RLEV1) cands.push_back({512, 2048, 0, 4});  // pipelined walker + helpers
+    cands.push_back({256, 4096, 0, 2});  // walker + expander shapes
     cands.push_back({128, 4096, 0, 2});
     if (codec == CODAG_RLEV2) {  // one-wave-per-chunk shapes
       cands.push_back({64, 4096, 0, 0});

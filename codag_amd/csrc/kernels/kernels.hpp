@@ -46,6 +46,13 @@ hipError_t launch_rlev1_pc(const uint8_t *comp, uint64_t comp_bytes, const uint6
                            uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t nexp,
                            hipStream_t stream);
 
+// RLE v1, software pipeline over 2 KiB windows: one walker wave and three
+// helper waves per chunk (chunk_bytes / w <= 2^18).
+hipError_t launch_rlev1_pipe(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
+                             const uint64_t *col_len, uint64_t n_chunks, uint32_t w,
+                             uint64_t chunk_bytes, uint8_t *out, int32_t *status,
+                             hipStream_t stream);
+
 // RLE v2, one walker wave and nexp (1 or 3) expander waves per chunk.
 hipError_t launch_rlev2_pc(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
                            const uint64_t *col_len, uint64_t n_chunks, uint32_t w,

@@ -128,7 +128,7 @@ def test_v1_error_status_matches_oracle(ctx, k):
     assert rc != 0 and st.tolist() == [rc]
 
 
-@pytest.mark.parametrize("shape", [(256, 4096, 2), (128, 4096, 2)])
+@pytest.mark.parametrize("shape", [(512, 2048, 4), (256, 4096, 2), (128, 4096, 2)])
 def test_v1_walker_and_expander_kernels(ctx, shape):
     # the walker + expander kernel on the synthetic columns, the known answers,
     # the malformed corpus, small chunks, and randomly damaged streams whose
@@ -166,6 +166,43 @@ def test_v1_walker_and_expander_kernels(ctx, shape):
         comp = np.frombuffer(b"".join(streams) + b"\0" * 16, np.uint8)
         st, _ = _check_chunks(ctx, ca.RLEV1, comp, off, cl, 8, 200 * 8)
         assert len(set(st.tolist())) > 2  # the damage produced several kinds of status
+    finally:
+        ctx.tune_set(ca.RLEV1, saved)
+
+
+@pytest.mark.parametrize("width", [8, 4, 1])
+def test_v1_pipeline_damaged_multiwindow_chunks(ctx, width):
+    # 128 KiB chunks whose streams span many 2 KiB windows, damaged by single
+    # bytes, by runs of continuation bytes (overlong varints that run past the
+    # indexed windows) and by truncation; status must equal the oracle's and
+    # undamaged chunks must decode exactly
+    say(f"rlev1 pipeline damaged multi-window chunks w={width}")
+    saved = ctx.tune_get(ca.RLEV1)
+    ctx.tune_set(ca.RLEV1, ca.TuneParams(512, 2048, 0, 4))
+    try:
+        raw, cont, blk, cl, n = ca.synth_rlev1(6 * CH // width, width, CH, 1 << 16, seed=40 + width)
+        comp = cont[8:]
+        rng = np.random.default_rng(50 + width)
+        streams = []
+        for t in range(48):
+            c = t % n
+            s = bytearray(comp[int(blk[c]): int(blk[c]) + int(cl[c])])
+            if t % 8 != 7:
+                for _ in range(int(rng.integers(1, 3))):
+                    k = int(rng.integers(0, len(s)))
+                    if t % 4 == 0:
+                        s[k:k + 3000] = b"\xff" * min(3000, len(s) - k)
+                    else:
+                        s[k] = int(rng.integers(0, 256))
+                if t % 5 == 0:
+                    s = s[: int(rng.integers(0, len(s) + 1))]
+            # padding between streams: every alignment of the stream start
+            streams.append((bytes(s), bytes(int(rng.integers(0, 16)))))
+        off = np.cumsum([0] + [len(b) + len(p) for b, p in streams])[:-1].astype(np.uint64)
+        cln = np.array([len(b) for b, _ in streams], np.uint64)
+        comp2 = np.frombuffer(b"".join(b + p for b, p in streams) + bytes(16), np.uint8)
+        st, _ = _check_chunks(ctx, ca.RLEV1, comp2, off, cln, width, CH)
+        assert (st == 0).sum() >= 6 and len(set(st.tolist())) > 2
     finally:
         ctx.tune_set(ca.RLEV1, saved)
 
