@@ -49,6 +49,7 @@ constexpr uint32_t kWin = 2048;            // window bytes
 constexpr uint32_t kBlocks = kWin / 16;    // 16-byte blocks per window
 constexpr uint32_t kHelpers = 7;
 constexpr uint32_t kThreads = kWave * (1 + kHelpers);
+static_assert(kHelpers == 7, "expansion passes are dealt in rounds of 1 + 1 + 5 * 2");
 constexpr uint32_t kRing = 4 * kWin;       // LDS byte ring
 constexpr uint32_t kMirror = 32;           // copy of the ring's first bytes past its end
 constexpr uint32_t kTerms = kWin + 2;      // terminators of one window (<= kWin: window 0's byte
@@ -56,6 +57,7 @@ constexpr uint32_t kTerms = kWin + 2;      // terminators of one window (<= kWin
 constexpr uint32_t kDescs = kWin / 2 + 4;  // segments starting in one window (each spans >= 2 bytes)
 constexpr uint32_t kTabRegs = kWin / 2 / kWave;  // walker registers holding a window's ent
 constexpr uint32_t kMaxCap = 1u << 21;      // values per chunk (descriptor field)
+constexpr uint32_t kSpare = kTerms - 1;    // sel / ent entry written for non-terminator bytes
 constexpr uint32_t kNone = 0xffffffffu;
 
 // Error keys: (value index << 2 | priority) << 32 | window. The smallest key
@@ -67,7 +69,7 @@ constexpr uint64_t kNoKey = ~0ull;
 struct Lds {
   uint8_t sbuf[kRing + kMirror];
   uint16_t sel[3][kTerms];     // ring byte offset of each terminator
-  uint16_t ent[2][kWin];       // the two stream bytes after each terminator
+  uint16_t ent[2][kTerms];     // the two stream bytes after each terminator
   uint32_t desc[2][kDescs];    // segment: r - T_w (11 bits) | first value index << 11
   uint8_t bcnt[2][kBlocks];    // terminators per 16-byte block
   uint32_t T[8];               // T[j & 7]: number of window j's first terminator
@@ -254,11 +256,13 @@ rlev1_pipe_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   }
         r = uni(r);
         o = uni(o);
+        if (r < Tfast && o < cap) {
 #pragma clang loop unroll(disable)
-        while (r < Tfast && o < cap) {
-          const uint32_t rl = r - Tw;
-          const uint32_t e = CODAG_ENT(rl);
-          CODAG_STEP(e)
+          do {  // the step ends the loop once the chunk is full
+            const uint32_t rl = r - Tw;
+            const uint32_t e = CODAG_ENT(rl);
+            CODAG_STEP(e)
+          } while (r < Tfast);
         }
 #pragma clang loop unroll(disable)
         while (!ovf && r < Tn && o < cap) {  // the stream's last two terminators
@@ -305,16 +309,16 @@ rlev1_pipe_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
       {
         const int32_t w = (int32_t)k - 1;
         if (w >= 0 && (uint32_t)w < nwin) {
-          const uint32_t c_lo = L.bcnt[w & 1][lane], c_hi = L.bcnt[w & 1][kWave + lane];
-          const uint32_t s_lo = wave_incl_scan(c_lo), s_hi = wave_incl_scan(c_hi);
-          const uint32_t tot_lo = uni(__shfl(s_lo, kWave - 1, kWave));
-          const uint32_t tot_hi = uni(__shfl(s_hi, kWave - 1, kWave));
-          const uint32_t Tw = uni(L.T[w & 7]);
-          if (h == 0 && lane == 0) L.T[(w + 1) & 7] = Tw + tot_lo + tot_hi;
           if (loader) {
+            // block b's first terminator: prefix over the window's block counts
+            const uint32_t c_lo = L.bcnt[w & 1][lane], c_hi = L.bcnt[w & 1][kWave + lane];
+            const uint32_t s_lo = wave_scan_dpp(c_lo), s_hi = wave_scan_dpp(c_hi);
+            const uint32_t tot_lo = __builtin_amdgcn_readlane(s_lo, kWave - 1);
+            const uint32_t tot_hi = __builtin_amdgcn_readlane(s_hi, kWave - 1);
+            if (gl == 0) L.T[(w + 1) & 7] = uni(L.T[w & 7]) + tot_lo + tot_hi;
             const uint32_t slot = w & 3;
             const uint32_t b = gl;
-            uint32_t idx = b < kWave ? s_lo - c_lo : tot_lo + s_hi - c_hi;
+            const uint32_t base = b < kWave ? s_lo - c_lo : tot_lo + s_hi - c_hi;
             uint32_t dn = __shfl_down(y.x, 1, kWave);
             if (lane == kWave - 1)
               dn = b == kBlocks - 1
@@ -327,15 +331,13 @@ rlev1_pipe_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
             uint16_t *ent = L.ent[w & 1];
             const uint32_t xo = slot * kWin + 16 * b;
             const uint32_t by[5] = {y.x, y.y, y.z, y.w, dn};
+            // branch-free: bytes that are no terminator write the spare entry
 #pragma unroll
             for (uint32_t j = 0; j < 16; j++) {
-              if (ym & (1u << j)) {
-                sel[idx] = (uint16_t)(xo + j);
-                const uint32_t q = (j + 1) >> 2, sh = (j + 1) & 3;
-                const uint32_t two = sh == 0 ? by[q] : __builtin_amdgcn_alignbyte(by[q + 1], by[q], sh);
-                ent[idx] = (uint16_t)two;
-                idx++;
-              }
+              const uint32_t ix = (ym >> j) & 1 ? base + __popc(ym & ((1u << j) - 1)) : kSpare;
+              const uint32_t q = (j + 1) >> 2, sh = (j + 1) & 3;
+              sel[ix] = (uint16_t)(xo + j);
+              ent[ix] = (uint16_t)(sh == 0 ? by[q] : __builtin_amdgcn_alignbyte(by[q + 1], by[q], sh));
             }
           }
         } else if (w >= 0 && h == 0 && lane == 0) {
@@ -372,7 +374,9 @@ rlev1_pipe_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
         const uint32_t nk = uni(L.T[(w + 2) & 7]) - Tw;  // ... of windows w and w+1 (indexed)
         const uint32_t bad_pr = (uint32_t)w + 2 >= nwin ? kPrTrunc : kPrPending;
         const uint16_t *sel0 = L.sel[w % 3];
-        const uint16_t *sel1 = L.sel[(w + 1) % 3];
+        // ring position of local terminator q: window w's entries, then w+1's
+        const uint16_t *sel1m = L.sel[(w + 1) % 3] - n0;
+        auto sel = [&](uint32_t q) -> uint32_t { return (q < n0 ? sel0 : sel1m)[q]; };
         const uint32_t *desc = L.desc[w & 1];
         // lanes exchange segment starts through these bytes (one wave's LDS
         // operations complete in order; a compiler barrier keeps the read a
@@ -380,9 +384,14 @@ rlev1_pipe_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
         uint8_t *mk = L.mark[h];
         const uint32_t npass = (o_end - o_first + kWave - 1) / kWave;
         uint32_t cur = 0;
-        // passes of 64 consecutive values, round-robin over the helpers
+        // passes of 64 consecutive values: of every 12, helpers 0 and 1
+        // (which also load, stage and index) take one each, the others two
+        const uint32_t per = h < 2 ? 1u : 2u;
+        const uint32_t first = h < 2 ? h : 2 * h - 2;
 #pragma clang loop unroll(disable)
-        for (uint32_t p = h; p < npass; p += kHelpers) {
+        for (uint32_t t = 0;; t++) {
+          const uint32_t p = 12 * (t / per) + first + (t % per);
+          if (p >= npass) break;
           const uint32_t ob = o_first + kWave * p;
           // cur: the last segment starting at or before value ob; lane i
           // holds segment cur + i
@@ -407,35 +416,31 @@ rlev1_pipe_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                              (uint32_t)((starts >> lane) & 1);
           const uint32_t dsc = __shfl(di, (int)j, kWave);
           const uint32_t v = ob + lane;
-          if (v < o_end) {
-            const uint32_t rl = dsc & 0x7ff;
-            const uint32_t os = dsc >> 11;
-            const uint32_t ii = v - os;
-            // ring position of local terminator q (window w or w+1)
-            auto sel = [&](uint32_t q) -> uint32_t { return q < n0 ? sel0[q] : sel1[q - n0]; };
-            const uint32_t s0 = sel0[rl];  // terminator before the header
-            const uint32_t hb = L.sbuf[s0 + 1], db = L.sbuf[s0 + 2];
-            const uint32_t run = hb < 0x80;
-            // run (decoder.h:736-790): the base ends on terminator rl + 2 +
-            // [delta byte < 0x80] and belongs to the run's first value;
-            // literal group (decoder.h:686-733): varint ii ends on terminator
-            // rl + 1 + ii and starts after terminator rl + ii (after the header
-            // for ii = 0)
-            const uint32_t qb = run ? rl + 2 + (db < 0x80) : rl + 1 + ii;
-            const uint32_t vidx = run ? os : v;
-            if (qb >= nk) {
-              report(L, vidx, bad_pr, (uint32_t)w);
-            } else {
-              const uint32_t start = run ? s0 + 3 : (ii == 0 ? s0 + 2 : sel(qb - 1) + 1);
-              const uint32_t n = ((sel(qb) - start) & (kRing - 1)) + 1;
-              if (n > 10) {  // longer than any 64-bit varint
-                report(L, vidx, kPrCorrupt, (uint32_t)w);
-              } else if (v < cap) {
-                uint64_t val = ring_varint<W>(L.sbuf, start, n);
-                if (run) val += (uint64_t)((int64_t)ii * (int32_t)(int8_t)db);  // delta byte
-                store_val<W>(dst, v, val);
-              }
+          const bool act = v < o_end;
+          const uint32_t rl = dsc & 0x7ff;
+          const uint32_t os = dsc >> 11;
+          const uint32_t ii = act ? v - os : 0u;  // lanes past the window read in range
+          const uint32_t s0 = sel0[rl];  // terminator before the header
+          const uint32_t hb = L.sbuf[s0 + 1], db = L.sbuf[s0 + 2];
+          const bool run = hb < 0x80;
+          // run (decoder.h:736-790): the base starts after the delta byte,
+          // ends on terminator rl + 2 + [delta byte < 0x80] and belongs to the
+          // run's first value; literal group (decoder.h:686-733): varint ii
+          // ends on terminator rl + 1 + ii and starts after terminator rl + ii
+          // (after the header for ii = 0)
+          const uint32_t qb = run ? rl + 2 + (db < 0x80) : rl + 1 + ii;
+          const bool complete = qb < nk;
+          const uint32_t qa = (run || ii == 0) ? rl : qb - 1;
+          const uint32_t start = sel(qa) + 1 + (run ? 2u : (ii == 0 ? 1u : 0u));
+          const uint32_t n = ((sel(complete ? qb : qa) - start) & (kRing - 1)) + 1;
+          if (act && complete && n <= 10) {
+            if (v < cap) {
+              uint64_t val = ring_varint<W>(L.sbuf, start, n);
+              if (run) val += (uint64_t)((int64_t)ii * (int32_t)(int8_t)db);  // delta byte
+              store_val<W>(dst, v, val);
             }
+          } else if (act) {  // incomplete or longer than any 64-bit varint
+            report(L, run ? os : v, complete ? kPrCorrupt : bad_pr, (uint32_t)w);
           }
         }
       }
