@@ -122,11 +122,38 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
   s.pdone = uni(s.pdone);
   s.pstop = uni(s.pstop);
   s.pfast = uni(s.pfast);
+  // A SHORT_REPEAT block advances p by at most 9 bytes and o by at most 10
+  // values: the paired fast path checks its bounds once per two blocks.
+  const uint32_t pf2 = uni(s.pfast >= 9 ? s.pfast - 9 : 0u);
+  const uint32_t of2 = uni(s.ofast >= 10 ? s.ofast - 10 : 0u);
   while (nd < (uint32_t)kWave) {
     // Fast path for runs of SHORT_REPEAT blocks (the most frequent header):
     // while the position and output index are far enough from the window
     // end and the chunk end that no bound can be hit, a block costs one
     // header read, two bit-field extracts and the lane record.
+#define CODAG_SR_STEP                                                                        \
+  {                                                                                          \
+    const uint32_t first = ((uint32_t)__builtin_amdgcn_readlane(vw, (p - vbase) >> 2) >>     \
+                            ((p & 3) * 8)) & 0xff;                                           \
+    if (first >= 64) break;                                                                  \
+    if (lane == nd) {                                                                        \
+      dpos = p;                                                                              \
+      dout = o;                                                                              \
+    }                                                                                        \
+    nd++;                                                                                    \
+    o += (first & 7) + 3;                                                                    \
+    p += 2 + ((first >> 3) & 7);                                                             \
+  }
+    for (;;) {  // two blocks per bounds check; the mirror covers p + 9
+      if (nd + 2 > (uint32_t)kWave || p >= pf2 || o >= of2) break;
+      if (p - vbase > 215) {
+        vbase = p & ~3u;
+        vw = sdw[vbase / 4 + lane];
+      }
+      CODAG_SR_STEP
+      CODAG_SR_STEP
+    }
+#undef CODAG_SR_STEP
     for (;;) {
       if (nd >= (uint32_t)kWave || p >= s.pfast || o >= s.ofast) break;
       if (p - vbase > 224) {
@@ -358,7 +385,8 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   uint8_t *dst = out + chunk * chunk_bytes;
   if (threadIdx.x == 0) L.bad = 0;
 
-  if (wave == 0) {  // ---- walker
+  if (wave == 0) {  // ---- walker: the critical path; its instructions issue first
+    __builtin_amdgcn_s_setprio(3);
     Walk s;
     s.src = comp + boff;
     s.len = (uint32_t)len;
