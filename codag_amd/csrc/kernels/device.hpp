@@ -40,6 +40,11 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
+// Number of set bits of m in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Inclusive wave64 prefix sum with DPP row shifts and row broadcasts: six
 // VALU operations, no LDS traffic.
 __device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {

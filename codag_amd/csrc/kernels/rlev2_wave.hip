@@ -1,10 +1,14 @@
 // rlev2_wave.hip -- RLE v2 decode kernels built around a scalar header walk.
 //
 // The chunk's stream is staged into an LDS window (kWin bytes). The header
-// walk runs in scalar registers: the 256 bytes around the walk position are
-// mirrored in one VGPR (lane l holds dword l) and header bytes are read with
-// v_readlane, so walking a block costs no memory round trip. A walk collects
-// up to 64 blocks (lane j <- block j); expansion then writes them:
+// walk is a serial chain (a block's size gives the next header), run by one
+// wave in span steps: every lane sizes the block a header at p + lane would
+// start, and the scalar unit follows the chain through those sizes with one
+// v_readlane per block; a wave prefix sum places the chained blocks' values.
+// Headers the span step does not take (errors, window edges, long DELTA
+// varints) go through a one-block scalar path that reads them from a VGPR
+// mirror of the 256 bytes around the walk position. A walk collects up to 64
+// block descriptors in LDS; expansion then writes them:
 //   * SHORT_REPEAT blocks (3..10 values) expand one lane per block;
 //   * DIRECT / DELTA / PATCHED_BASE blocks expand across a wave (expand_big:
 //     one lane per value, wave prefix sums for DELTA, a wave scan of the
@@ -61,15 +65,13 @@ struct Walk {
   const uint8_t *src;  // chunk stream
   uint32_t len;        // stream bytes
   uint32_t cap;        // values in the chunk
-  uint32_t ofast;      // o < ofast keeps a SHORT_REPEAT block inside the chunk
   uint32_t wpos;       // stream offset held at sbuf[lead]
   uint32_t lead, P;    // window: bytes [lead, P) of sbuf hold the stream
   uint32_t at_end;     // the window holds the stream's end
-  // The walk stops at p >= pdone (stream consumed: P when the window holds
-  // the stream's end), restages at p > pstop (header bytes might lie past
-  // the window: P - 32 when it does not), and takes the SHORT_REPEAT fast
-  // path while p < pfast (a block is at most 10 bytes).
-  uint32_t pdone, pstop, pfast;
+  // The one-block path stops at p >= pdone (stream consumed: P when the
+  // window holds the stream's end) and restages at p > pstop (header bytes
+  // might lie past the window: P - 32 when it does not).
+  uint32_t pdone, pstop;
   uint32_t p, o;       // walk position (LDS offset) and values produced
   uint32_t stop;       // outcome of the last walk_batch (kStop*)
   int32_t err;         // error status of the last walk_batch
@@ -89,7 +91,6 @@ __device__ __forceinline__ void walk_stage(Walk &s, uint8_t *sbuf, uint32_t pos,
   s.wpos = pos;
   s.pdone = s.at_end ? s.P : 0xffffffffu;
   s.pstop = s.at_end ? 0xffffffffu : s.P - 32;
-  s.pfast = s.at_end ? (s.P >= 10 ? s.P - 9 : 0) : s.P - 31;
   s.p = s.lead;
 }
 
@@ -101,13 +102,89 @@ __device__ __forceinline__ void walk_mirror(Walk &s, const uint8_t *sbuf, uint32
 
 enum : uint32_t { kStopNone = 0, kStopDone = 1, kStopRestage = 2 };
 
-// Collects up to 64 blocks: lane j receives block j's LDS offset (dpos) and
-// first value index (dout). Sets s.stop to kStopDone when the stream is
-// consumed or kStopRestage when the next header or block may lie past the
-// window, and s.err on an error. (Outcomes travel as integers in the walk
-// state: flags passed by reference would be kept in scratch memory.)
+constexpr uint32_t kSpan = kWave - 1;  // header positions one span step examines
+
+// Byte size and value count of the block whose header would start at LDS
+// offset q (decoder_new.h:1003-1183 read the same header fields), and whether
+// the span step may take it: the block ends inside the window (q + size <=
+// P) and is none of the cases left to the one-block path (a PATCHED_BASE
+// width sum past 64; a DELTA whose two varints do not end within the 14
+// bytes after its header, or one of them longer than 10 bytes). Bytes past P
+// never make a position acceptable: any field they feed puts the block's end
+// past P.
+__device__ __forceinline__ void span_block(const uint8_t *sbuf, uint32_t q, uint32_t P,
+                                           uint32_t &size, uint32_t &cnt, bool &ok) {
+  const uint32_t *d = reinterpret_cast<const uint32_t *>(sbuf) + (q >> 2);
+  const uint32_t d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
+  const uint32_t x0 = __builtin_amdgcn_alignbyte(d1, d0, q & 3);  // bytes q .. q+3
+  const uint32_t x1 = __builtin_amdgcn_alignbyte(d2, d1, q & 3);  // q+4 .. q+7
+  const uint32_t x2 = __builtin_amdgcn_alignbyte(d3, d2, q & 3);  // q+8 .. q+11
+  const uint32_t x3 = __builtin_amdgcn_alignbyte(d4, d3, q & 3);  // q+12 .. q+15
+  // select forms of decode_width / closest_fixed: per-lane selects, no
+  // divergent branches
+  auto width = [](uint32_t k) { return k < 24 ? k + 1 : (k < 28 ? 2 * k - 22 : 8 * k - 184); };
+  auto closest = [](uint32_t b) {  // b >= 2 here
+    return b <= 24 ? b : (b <= 32 ? (b + 1) & ~1u : (b + 7) & ~7u);
+  };
+  const uint32_t first = x0 & 0xff;
+  const uint32_t kind = first >> 6;
+  const uint32_t code = (first >> 1) & 0x1f;
+  const uint32_t fbw = width(code);
+  // the 9-bit length field plus one, kept in 8 bits (decoder_new.h:1018)
+  const uint32_t len8 = ((x0 >> 8) + 1) & 0xff;
+  const uint32_t body = (len8 * fbw + 7) >> 3;
+  // PATCHED_BASE: base width, patch width, gap width, patch list length
+  const uint32_t third = (x0 >> 16) & 0xff, fourth = x0 >> 24;
+  const uint32_t pw = width(third & 0x1f), pgw = (fourth >> 5) + 1;
+  const uint32_t pb_size =
+      5 + (third >> 5) + body + (((fourth & 0x1f) * closest(pw + pgw) + 7) >> 3);
+  // DELTA: the varints of base and delta base from byte q+2, looked for in
+  // the 14 bytes q+2 .. q+15 (tl: bytes 0..7 of that string, th: bytes
+  // 8..13); e1 / e2 index the bytes that end them
+  const uint64_t vl = ((uint64_t)x2 << 48) | ((uint64_t)x1 << 16) | (x0 >> 16);
+  const uint64_t vh = ((uint64_t)x3 << 16) | (x2 >> 16);
+  const uint64_t tl = ~vl & 0x8080808080808080ull;
+  const uint64_t th = ~vh & 0x0000808080808080ull;
+  const uint32_t e1 = tl ? (uint32_t)__builtin_ctzll(tl) >> 3
+                         : (th ? 8 + ((uint32_t)__builtin_ctzll(th) >> 3) : 14u);
+  const uint64_t tl2 = e1 < 7 ? tl & (~0ull << (8 * e1 + 8)) : 0ull;
+  const uint64_t th2 = e1 < 8 ? th : (e1 < 13 ? th & (~0ull << (8 * e1 - 56)) : 0ull);
+  const uint32_t e2 = tl2 ? (uint32_t)__builtin_ctzll(tl2) >> 3
+                          : (th2 ? 8 + ((uint32_t)__builtin_ctzll(th2) >> 3) : 14u);
+  const uint32_t nd8 = (len8 - 2) & 0xff;
+  const uint32_t delta_size = 3 + e2 + (code ? (nd8 * fbw + 7) >> 3 : 0u);
+  uint32_t sz = 2 + ((first >> 3) & 7), c = (first & 7) + 3;  // SHORT_REPEAT
+  bool good = true;
+  sz = kind == 1 ? 2 + body : sz;
+  sz = kind == 2 ? pb_size : sz;
+  sz = kind == 3 ? delta_size : sz;
+  c = kind == 0 ? c : (kind == 3 ? 2 + nd8 : len8);
+  good = kind == 2 ? pw + pgw <= 64 : good;
+  // a varint over 10 bytes is the one-block path's error to report
+  good = kind == 3 ? e2 < 14 && e1 < 10 && e2 - e1 <= 10 : good;
+  size = sz;
+  cnt = c;
+  ok = good && q + sz <= P;
+}
+
+// Collects up to 64 blocks into the LDS descriptor arrays: block j's LDS
+// offset goes to dpos[j] and its first value index to dout[j]. Sets s.stop
+// to kStopDone when the stream is consumed or kStopRestage when the next
+// header or block may lie past the window, and s.err on an error. (Outcomes
+// travel as integers in the walk state: flags passed by reference would be
+// kept in scratch memory.)
+//
+// Span step: each lane l < 63 sizes the block a header at p + l would start
+// (span_block), so the walk from p is a chain of lane indices -- next(l) = l
+// + size, lane 63 standing for "past the span" and a position the step does
+// not take pointing at itself. The scalar walker follows that chain with one
+// v_readlane per block; a wave prefix sum of the chained blocks' value counts
+// gives their output indices, and the chained lanes write their descriptors.
+// The first block that would overflow the chunk or the batch ends the step;
+// positions the step does not take go through the one-block path below,
+// which owns every error and window decision.
 __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uint32_t lane,
-                                               uint32_t &dpos, uint32_t &dout) {
+                                               uint32_t *ldpos, uint32_t *ldout) {
   const uint32_t *sdw = reinterpret_cast<const uint32_t *>(sbuf);
   uint32_t nd = 0;
   // The walk state is wave-uniform; saying so keeps the loop in scalar
@@ -118,62 +195,54 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
   int32_t err = kOk;
   s.P = uni(s.P);
   s.cap = uni(s.cap);
-  s.ofast = uni(s.ofast);
   s.pdone = uni(s.pdone);
   s.pstop = uni(s.pstop);
-  s.pfast = uni(s.pfast);
-  // A SHORT_REPEAT block advances p by at most 9 bytes and o by at most 10
-  // values: the paired fast path checks its bounds once per two blocks.
-  const uint32_t pf2 = uni(s.pfast >= 9 ? s.pfast - 9 : 0u);
-  const uint32_t of2 = uni(s.ofast >= 10 ? s.ofast - 10 : 0u);
   while (nd < (uint32_t)kWave) {
-    // Fast path for runs of SHORT_REPEAT blocks (the most frequent header):
-    // while the position and output index are far enough from the window
-    // end and the chunk end that no bound can be hit, a block costs one
-    // header read, two bit-field extracts and the lane record.
-#define CODAG_SR_STEP                                                                        \
-  {                                                                                          \
-    const uint32_t first = ((uint32_t)__builtin_amdgcn_readlane(vw, (p - vbase) >> 2) >>     \
-                            ((p & 3) * 8)) & 0xff;                                           \
-    if (first >= 64) break;                                                                  \
-    if (lane == nd) {                                                                        \
-      dpos = p;                                                                              \
-      dout = o;                                                                              \
-    }                                                                                        \
-    nd++;                                                                                    \
-    o += (first & 7) + 3;                                                                    \
-    p += 2 + ((first >> 3) & 7);                                                             \
-  }
-    for (;;) {  // two blocks per bounds check; the mirror covers p + 9
-      if (nd + 2 > (uint32_t)kWave || p >= pf2 || o >= of2) break;
-      if (p - vbase > 215) {
-        vbase = p & ~3u;
-        vw = sdw[vbase / 4 + lane];
+    {  // ---- span step over the headers at p .. p + 62
+      uint32_t sz, c;
+      bool ok;
+      span_block(sbuf, p + lane, s.P, sz, c, ok);
+      const uint32_t nx = lane + sz;  // next header, relative to p
+      const uint32_t tab = (lane < kSpan && ok) ? (nx < kSpan ? nx : kSpan) : lane;
+      // tab rises strictly along the chain until a fixed point: four links
+      // per test
+      uint32_t r = 0, rf = kSpan;
+      uint64_t mark = 0;
+      for (;;) {
+        const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane(tab, r);
+        const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane(tab, r1);
+        const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane(tab, r2);
+        const uint32_t r4 = (uint32_t)__builtin_amdgcn_readlane(tab, r3);
+        mark |= (1ull << r) | (1ull << r1) | (1ull << r2) | (1ull << r3);
+        if (r4 == r3) {
+          rf = r4;
+          break;
+        }
+        r = r4;
       }
-      CODAG_SR_STEP
-      CODAG_SR_STEP
+      mark &= ~(1ull << rf);  // the fixed point is not taken here
+      const bool blk = (mark >> lane) & 1;
+      const uint32_t cl = blk ? c : 0u;
+      const uint32_t incl = wave_scan_dpp(cl);
+      const uint32_t rank = lanes_below(mark);
+      // the first block past the chunk's values or the batch's 64 slots ends
+      // the step (the one-block path reports the overflow)
+      const uint64_t cut = __ballot(blk && (o + incl > s.cap || nd + rank >= (uint32_t)kWave));
+      if (cut) mark &= (1ull << __builtin_ctzll(cut)) - 1;
+      if (mark) {
+        if ((mark >> lane) & 1) {
+          ldpos[nd + rank] = p + lane;
+          ldout[nd + rank] = o + incl - cl;
+        }
+        const uint32_t last = 63 - (uint32_t)__builtin_clzll(mark);
+        o += (uint32_t)__builtin_amdgcn_readlane(incl, last);
+        p += (uint32_t)__builtin_amdgcn_readlane(nx, last);
+        nd += (uint32_t)__builtin_popcountll(mark);
+      }
+      if (nd >= (uint32_t)kWave) break;
+      if (rf == kSpan && !cut) continue;  // the chain left the span
     }
-#undef CODAG_SR_STEP
-    for (;;) {
-      if (nd >= (uint32_t)kWave || p >= s.pfast || o >= s.ofast) break;
-      if (p - vbase > 224) {
-        vbase = p & ~3u;
-        vw = sdw[vbase / 4 + lane];
-      }
-      const uint32_t k = (p - vbase) >> 2;
-      const uint32_t first = (uint32_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k) |
-                                         ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vw, k + 1)
-                                          << 32)) >> ((p & 3) * 8)) & 0xff;
-      if (first >= 64) break;
-      if (lane == nd) {
-        dpos = p;
-        dout = o;
-      }
-      nd++;
-      o += (first & 7) + 3;
-      p += 2 + ((first >> 3) & 7);
-    }
-    if (nd >= (uint32_t)kWave) break;
+    // ---- one-block path: the block at p, which the span step did not take
     if (p >= s.pdone) {  // decoder_new.h:1000 -- stream consumed
       stop = kStopDone;
       break;
@@ -238,9 +307,9 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
       err = kOverflow;
       break;
     }
-    if (lane == nd) {
-      dpos = p;
-      dout = o;
+    if (lane == 0) {
+      ldpos[nd] = p;
+      ldout[nd] = o;
     }
     nd++;
     o += cnt;
@@ -289,6 +358,8 @@ __device__ __forceinline__ bool chunk_ok(uint64_t boff, uint64_t len, uint64_t c
 
 template <uint32_t kWin> struct WaveLds {
   uint8_t sbuf[kWin + 288];  // window + slack for the 256-byte VGPR mirror
+  uint32_t dpos[kWave];      // the batch's block descriptors
+  uint32_t dout[kWave];
 };
 
 template <uint32_t kWin, uint32_t W>
@@ -310,7 +381,6 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   s.src = comp + boff;
   s.len = (uint32_t)len;
   s.cap = (uint32_t)(chunk_bytes / W);
-  s.ofast = s.cap >= 10 ? s.cap - 9 : 0;
   s.o = 0;
   uint8_t *dst = out + chunk * chunk_bytes;
 
@@ -321,9 +391,10 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   uint32_t bad = 0;
   bool done = (len == 0);
   while (!done && st == kOk) {
-    uint32_t dpos = 0, dout = 0;
     const uint32_t p0 = s.p;
-    const uint32_t nd = walk_batch(s, L.sbuf, lane, dpos, dout);
+    const uint32_t nd = walk_batch(s, L.sbuf, lane, L.dpos, L.dout);
+    asm volatile("" ::: "memory");  // descriptors written by other lanes of this wave
+    const uint32_t dpos = L.dpos[lane], dout = L.dout[lane];
     done = s.stop == kStopDone;
     const bool restage = s.stop == kStopRestage;
     st = s.err;
@@ -391,18 +462,17 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
     s.src = comp + boff;
     s.len = (uint32_t)len;
     s.cap = cap;
-    s.ofast = cap >= 10 ? cap - 9 : 0;
     s.o = 0;
     walk_stage<kWin>(s, L.sbuf, 0, lane);
     walk_mirror(s, L.sbuf, lane);  // the walker's own LDS writes precede its reads
     int32_t st = kOk;
     bool done = (len == 0);
     for (uint32_t k = 0;; k++) {
-      uint32_t dpos = 0, dout = 0, nd = 0;
+      uint32_t nd = 0;
       bool restage = false;
       if (!done && st == kOk) {
         const uint32_t p0 = s.p;
-        nd = walk_batch(s, L.sbuf, lane, dpos, dout);
+        nd = walk_batch(s, L.sbuf, lane, L.dpos[k & 1], L.dout[k & 1]);
         done = s.stop == kStopDone;
         restage = s.stop == kStopRestage;
         st = s.err;
@@ -410,8 +480,6 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
       }
       const bool last = done || st != kOk;
       if (last) restage = false;
-      L.dpos[k & 1][lane] = dpos;
-      L.dout[k & 1][lane] = dout;
       if (lane == 0) L.info[k & 1] = nd | (last ? kBatchLast : 0u) | (restage ? kBatchRestage : 0u);
       __syncthreads();  // B_k
       if (last) {
