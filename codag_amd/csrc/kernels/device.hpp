@@ -57,6 +57,38 @@ __device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
   return v;
 }
 
+// 64-bit form of wave_scan_dpp: each step moves both halves by DPP and adds
+// with carry.
+__device__ __forceinline__ uint64_t wave_scan_dpp64(uint64_t v) {
+#define CODAG_SCAN64_STEP(ctrl, rows)                                                   \
+  {                                                                                     \
+    const uint32_t tl = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v,      \
+                                                              ctrl, rows, 0xf, false);  \
+    const uint32_t th = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), \
+                                                              ctrl, rows, 0xf, false);  \
+    v += ((uint64_t)th << 32) | tl;                                                     \
+  }
+  CODAG_SCAN64_STEP(0x111, 0xf)  // row_shr:1
+  CODAG_SCAN64_STEP(0x112, 0xf)  // row_shr:2
+  CODAG_SCAN64_STEP(0x114, 0xf)  // row_shr:4
+  CODAG_SCAN64_STEP(0x118, 0xf)  // row_shr:8
+  CODAG_SCAN64_STEP(0x142, 0xa)  // row_bcast:15
+  CODAG_SCAN64_STEP(0x143, 0xc)  // row_bcast:31
+#undef CODAG_SCAN64_STEP
+  return v;
+}
+
+// OR of v over lanes 0..31, returned to every lane (DPP row scan, then the
+// row-0 total broadcast into row 1; read from lane 31).
+__device__ __forceinline__ uint32_t lanes32_or(uint32_t v) {
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+}
+
 __device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
   const uint32_t lane = lane_id();
 #pragma unroll

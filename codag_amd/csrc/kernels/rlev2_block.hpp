@@ -36,19 +36,6 @@ __device__ __forceinline__ uint32_t closest_fixed(uint32_t bits) {
   return (bits + 7) & ~7u;
 }
 
-// Unsigned LEB128 from LDS starting at *q (bits past 64 dropped); advances *q.
-__device__ __forceinline__ uint64_t lds_uvarint(const uint8_t *s, uint32_t *q) {
-  uint64_t v = 0;
-  uint32_t sh = 0;
-  uint8_t b;
-  do {
-    b = s[(*q)++];
-    if (sh < 64) v |= (uint64_t)(b & 0x7f) << sh;
-    sh += 7;
-  } while (b >= 0x80);
-  return v;
-}
-
 // MSB-first bit field of width f (1..32) starting at bit b of the LDS byte
 // string s: the field lies inside the two dwords at b/32 (byte-swapped into a
 // 64-bit big-endian window). s must extend 8 bytes past the field's dword.
@@ -101,27 +88,48 @@ __device__ __forceinline__ typename Col<W>::T be_bytes(const uint8_t *s, uint32_
 
 // Expands the DIRECT / DELTA / PATCHED_BASE block at LDS offset p into
 // element o onwards of dst. All 64 lanes of the wave must call it (cross-lane
-// scans). Sets *bad for a patch outside its block.
+// scans). Sets *bad for a patch outside its block. The block was accepted by
+// the header walk, so its two DELTA varints are at most 10 bytes each.
 template <uint32_t W>
 __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint32_t o, uint8_t *dst,
                                            uint32_t lane, uint32_t *bad) {
   using T = typename Col<W>::T;
   constexpr T mask = Col<W>::kMask;
   constexpr uint32_t nbits = 8 * W;
-  const uint32_t first = sbuf[p];
+  // the four header bytes in one uniform LDS read
+  const uint32_t *hd = reinterpret_cast<const uint32_t *>(sbuf) + (p >> 2);
+  const uint32_t x0 = uni(__builtin_amdgcn_alignbyte(hd[1], hd[0], p & 3));
+  const uint32_t first = x0 & 0xff;
   const uint32_t kind = first >> 6;
   const uint32_t code = (first >> 1) & 0x1f;
   const uint32_t fbw = decode_width(code);
-  const uint32_t len8 = ((((first & 1) << 8) | sbuf[p + 1]) + 1) & 0xff;
+  const uint32_t len8 = ((x0 >> 8) + 1) & 0xff;  // 9-bit length + 1, kept in 8 bits
   if (kind == 1) {  // DIRECT (decoder_new.h:1016-1044)
     const uint32_t b0 = (p + 2) * 8;
     for (uint32_t i = lane; i < len8; i += kWave)
       store_w<W>(dst, o + i, field<W>(sbuf, b0 + i * fbw, fbw) & mask);
   } else if (kind == 3) {  // DELTA (decoder_new.h:1045-1093)
-    uint32_t q = p + 2;
-    const T base = (T)lds_uvarint(sbuf, &q) & mask;
-    const int64_t zz = (int64_t)lds_uvarint(sbuf, &q);
+    // Lane i < 32 holds byte p + i. The varints of base and delta base start
+    // at byte p + 2 and end at the first bytes below 0x80 (e1, e2); each
+    // lane shifts its 7 value bits into place and DPP OR-reductions assemble
+    // the varints (decoder_new.h:978-989; a 10th byte keeps only its lowest
+    // bit, at bit 63).
+    const uint32_t b = lane < 32 ? sbuf[p + lane] : 0x80u;
+    const uint64_t term = __ballot(lane >= 2 && b < 0x80);
+    const uint32_t e1 = uni((uint32_t)__builtin_ctzll(term));
+    const uint32_t e2 = uni((uint32_t)__builtin_ctzll(term & ~((2ull << e1) - 1)));
+    const bool inb = lane >= 2 && lane <= e1;
+    const bool inz = lane > e1 && lane <= e2;
+    const uint32_t sh = inb ? 7 * (lane - 2) : (inz ? 7 * (lane - e1 - 1) : 0u);  // <= 63
+    const uint64_t g = (uint64_t)(b & 0x7f) << sh;
+    const uint64_t gb = inb ? g : 0ull, gz = inz ? g : 0ull;
+    uint64_t base64 = lanes32_or((uint32_t)gb);
+    if (W > 4) base64 |= (uint64_t)lanes32_or((uint32_t)(gb >> 32)) << 32;
+    const int64_t zz = (int64_t)(((uint64_t)lanes32_or((uint32_t)(gz >> 32)) << 32) |
+                                 lanes32_or((uint32_t)gz));
+    const T base = (T)base64 & mask;
     const T bd = (T)((zz >> 1) ^ -(zz & 1)) & mask;  // arithmetic >>
+    const uint32_t q = p + e2 + 1;                   // first packed delta
     const uint32_t nd8 = (len8 - 2) & 0xff;
     if (lane == 0) store_w<W>(dst, o, base);
     if (lane == 1) store_w<W>(dst, o + 1, (T)(base + bd) & mask);
@@ -136,16 +144,19 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
         const uint32_t i = c + lane;
         const T d = i < nd8 ? (field<W>(sbuf, b0 + i * fbw, fbw) & mask) : (T)0;
         T s;
-        if (W <= 4) s = (T)wave_incl_scan((uint32_t)d);
-        else s = (T)wave_incl_scan64((uint64_t)d);
+        if (W <= 4) s = (T)wave_scan_dpp((uint32_t)d);
+        else s = (T)wave_scan_dpp64((uint64_t)d);
         const T v = neg ? (T)(carry - s) : (T)(carry + s);
         if (i < nd8) store_w<W>(dst, o + 2 + i, v & mask);
-        const T tot = (T)__shfl(s, kWave - 1, kWave);
+        T tot;
+        if (W <= 4) tot = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)s, kWave - 1);
+        else tot = (T)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)s >> 32), kWave - 1) << 32) |
+                       (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)s, kWave - 1));
         carry = neg ? (T)(carry - tot) : (T)(carry + tot);
       }
     }
   } else {  // PATCHED_BASE (decoder_new.h:1094-1183)
-    const uint32_t third = sbuf[p + 2], fourth = sbuf[p + 3];
+    const uint32_t third = (x0 >> 16) & 0xff, fourth = x0 >> 24;
     const uint32_t bw = (third >> 5) + 1;
     const uint32_t pw = decode_width(third & 0x1f);
     const uint32_t pgw = (fourth >> 5) + 1;
@@ -158,17 +169,17 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
     // lane j < pll: entry j -> running gap (wave scan) and its OR-term
     const uint64_t ent = lane < pll ? bits64(sbuf, pb0 + lane * cfb, cfb) : 0;
     const uint32_t g = lane < pll ? (uint32_t)(pw >= 64 ? 0 : (ent >> pw)) : 0;
-    const uint32_t gsum = wave_incl_scan(g);
+    const uint32_t gsum = wave_scan_dpp(g);
     const T add = (fbw >= nbits) ? (T)0 : (T)(((ent & pmask) << fbw) & (uint64_t)mask);
     if (lane < pll && gsum >= len8) *bad = 1;
-    // all lanes stay active around the cross-lane reads (a shuffle from an
-    // inactive lane returns 0)
     for (uint32_t c = 0; c < len8; c += kWave) {
       const uint32_t i = c + lane;
       T v = i < len8 ? (field<W>(sbuf, vb0 + i * fbw, fbw) & mask) : (T)0;
-      for (uint32_t j = 0; j < pll; j++) {
-        const uint32_t gj = __shfl(gsum, j, kWave);
-        const T aj = (T)__shfl(add, j, kWave);
+      for (uint32_t j = 0; j < pll; j++) {  // patch entries read from their lanes
+        const uint32_t gj = (uint32_t)__builtin_amdgcn_readlane((int)gsum, (int)j);
+        T aj = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)add, (int)j);
+        if (W > 4)
+          aj |= (T)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)add >> 32), (int)j) << 32);
         if (gj == i) v |= aj;
       }
       if (i < len8) store_w<W>(dst, o + i, (T)(v + base) & mask);
