@@ -45,13 +45,14 @@ __device__ __forceinline__ uint32_t bits32(const uint8_t *s, uint32_t b, uint32_
   return (uint32_t)((v << (b & 31)) >> (64 - f));
 }
 
-// Same for widths 33..64: three dwords.
+// Same for widths 33..64: three dwords, read together (no branch on the bit
+// offset, so the three loads share one wait).
 __device__ __forceinline__ uint64_t bits64(const uint8_t *s, uint32_t b, uint32_t f) {
   const uint32_t *d = reinterpret_cast<const uint32_t *>(s) + (b >> 5);
   const uint32_t r = b & 31;
   const uint64_t ab = ((uint64_t)__builtin_bswap32(d[0]) << 32) | __builtin_bswap32(d[1]);
   const uint32_t c = __builtin_bswap32(d[2]);
-  const uint64_t top = r ? ((ab << r) | (c >> (32 - r))) : ab;  // 64 bits from bit b on
+  const uint64_t top = (ab << r) | ((((uint64_t)c) << r) >> 32);  // 64 bits from bit b on
   return f >= 64 ? top : (top >> (64 - f));
 }
 
@@ -90,15 +91,24 @@ __device__ __forceinline__ typename Col<W>::T be_bytes(const uint8_t *s, uint32_
 // element o onwards of dst. All 64 lanes of the wave must call it (cross-lane
 // scans). Sets *bad for a patch outside its block. The block was accepted by
 // the header walk, so its two DELTA varints are at most 10 bytes each.
+//
+// LDS reads are issued unconditionally (lanes past a block's count read its
+// element 0) and two 64-element groups at a time, so reads that do not depend
+// on each other share one wait.
 template <uint32_t W>
 __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint32_t o, uint8_t *dst,
                                            uint32_t lane, uint32_t *bad) {
   using T = typename Col<W>::T;
   constexpr T mask = Col<W>::kMask;
   constexpr uint32_t nbits = 8 * W;
-  // the four header bytes in one uniform LDS read
-  const uint32_t *hd = reinterpret_cast<const uint32_t *>(sbuf) + (p >> 2);
-  const uint32_t x0 = uni(__builtin_amdgcn_alignbyte(hd[1], hd[0], p & 3));
+  // Lane l reads dword p/4 + (l & 7) and byte p + (l & 31): the header,
+  // PATCHED_BASE's base value and DELTA's varints come from these two reads.
+  const uint32_t hw = reinterpret_cast<const uint32_t *>(sbuf)[(p >> 2) + (lane & 7)];
+  const uint32_t hb = sbuf[p + (lane & 31)];
+  const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)hw, 0);
+  const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)hw, 1);
+  const uint32_t sh = 8 * (p & 3);
+  const uint32_t x0 = (uint32_t)((((uint64_t)w1 << 32) | w0) >> sh);  // bytes p .. p+3
   const uint32_t first = x0 & 0xff;
   const uint32_t kind = first >> 6;
   const uint32_t code = (first >> 1) & 0x1f;
@@ -106,22 +116,27 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
   const uint32_t len8 = ((x0 >> 8) + 1) & 0xff;  // 9-bit length + 1, kept in 8 bits
   if (kind == 1) {  // DIRECT (decoder_new.h:1016-1044)
     const uint32_t b0 = (p + 2) * 8;
-    for (uint32_t i = lane; i < len8; i += kWave)
-      store_w<W>(dst, o + i, field<W>(sbuf, b0 + i * fbw, fbw) & mask);
+    for (uint32_t c = 0; c < len8; c += 2 * kWave) {
+      const uint32_t i0 = c + lane, i1 = c + kWave + lane;
+      const T v0 = field<W>(sbuf, b0 + (i0 < len8 ? i0 : 0u) * fbw, fbw) & mask;
+      const T v1 = field<W>(sbuf, b0 + (i1 < len8 ? i1 : 0u) * fbw, fbw) & mask;
+      if (i0 < len8) store_w<W>(dst, o + i0, v0);
+      if (i1 < len8) store_w<W>(dst, o + i1, v1);
+    }
   } else if (kind == 3) {  // DELTA (decoder_new.h:1045-1093)
-    // Lane i < 32 holds byte p + i. The varints of base and delta base start
-    // at byte p + 2 and end at the first bytes below 0x80 (e1, e2); each
-    // lane shifts its 7 value bits into place and DPP OR-reductions assemble
-    // the varints (decoder_new.h:978-989; a 10th byte keeps only its lowest
-    // bit, at bit 63).
-    const uint32_t b = lane < 32 ? sbuf[p + lane] : 0x80u;
+    // The varints of base and delta base start at byte p + 2 and end at the
+    // first bytes below 0x80 (e1, e2); each lane shifts its byte's 7 value
+    // bits into place and DPP OR-reductions assemble the varints
+    // (decoder_new.h:978-989; a 10th byte keeps only its lowest bit, at bit
+    // 63).
+    const uint32_t b = lane < 32 ? hb : 0x80u;
     const uint64_t term = __ballot(lane >= 2 && b < 0x80);
     const uint32_t e1 = uni((uint32_t)__builtin_ctzll(term));
     const uint32_t e2 = uni((uint32_t)__builtin_ctzll(term & ~((2ull << e1) - 1)));
     const bool inb = lane >= 2 && lane <= e1;
     const bool inz = lane > e1 && lane <= e2;
-    const uint32_t sh = inb ? 7 * (lane - 2) : (inz ? 7 * (lane - e1 - 1) : 0u);  // <= 63
-    const uint64_t g = (uint64_t)(b & 0x7f) << sh;
+    const uint32_t vsh = inb ? 7 * (lane - 2) : (inz ? 7 * (lane - e1 - 1) : 0u);  // <= 63
+    const uint64_t g = (uint64_t)(b & 0x7f) << vsh;
     const uint64_t gb = inb ? g : 0ull, gz = inz ? g : 0ull;
     uint64_t base64 = lanes32_or((uint32_t)gb);
     if (W > 4) base64 |= (uint64_t)lanes32_or((uint32_t)(gb >> 32)) << 32;
@@ -129,7 +144,6 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
                                  lanes32_or((uint32_t)gz));
     const T base = (T)base64 & mask;
     const T bd = (T)((zz >> 1) ^ -(zz & 1)) & mask;  // arithmetic >>
-    const uint32_t q = p + e2 + 1;                   // first packed delta
     const uint32_t nd8 = (len8 - 2) & 0xff;
     if (lane == 0) store_w<W>(dst, o, base);
     if (lane == 1) store_w<W>(dst, o + 1, (T)(base + bd) & mask);
@@ -139,20 +153,29 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
     } else {
       const bool neg = (bd == 0);  // multiplier rule, decoder_new.h:1057
       T carry = base + bd;
-      const uint32_t b0 = q * 8;
-      for (uint32_t c = 0; c < nd8; c += kWave) {  // uniform loop: all lanes scan
-        const uint32_t i = c + lane;
-        const T d = i < nd8 ? (field<W>(sbuf, b0 + i * fbw, fbw) & mask) : (T)0;
-        T s;
-        if (W <= 4) s = (T)wave_scan_dpp((uint32_t)d);
-        else s = (T)wave_scan_dpp64((uint64_t)d);
-        const T v = neg ? (T)(carry - s) : (T)(carry + s);
+      const uint32_t b0 = (p + e2 + 1) * 8;  // first packed delta
+      auto step = [&](uint32_t i, T d) {     // wave prefix sum of one group
+        T sc;
+        if (W <= 4) sc = (T)wave_scan_dpp((uint32_t)d);
+        else sc = (T)wave_scan_dpp64((uint64_t)d);
+        const T v = neg ? (T)(carry - sc) : (T)(carry + sc);
         if (i < nd8) store_w<W>(dst, o + 2 + i, v & mask);
         T tot;
-        if (W <= 4) tot = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)s, kWave - 1);
-        else tot = (T)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)s >> 32), kWave - 1) << 32) |
-                       (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)s, kWave - 1));
+        if (W <= 4) {
+          tot = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)sc, kWave - 1);
+        } else {
+          const uint64_t s64 = (uint64_t)sc;
+          tot = (T)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(s64 >> 32), kWave - 1) << 32) |
+                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)s64, kWave - 1));
+        }
         carry = neg ? (T)(carry - tot) : (T)(carry + tot);
+      };
+      for (uint32_t c = 0; c < nd8; c += 2 * kWave) {  // uniform loop: all lanes scan
+        const uint32_t i0 = c + lane, i1 = c + kWave + lane;
+        const T r0 = field<W>(sbuf, b0 + (i0 < nd8 ? i0 : 0u) * fbw, fbw) & mask;
+        const T r1 = field<W>(sbuf, b0 + (i1 < nd8 ? i1 : 0u) * fbw, fbw) & mask;
+        step(i0, i0 < nd8 ? r0 : (T)0);
+        if (c + kWave < nd8) step(i1, i1 < nd8 ? r1 : (T)0);
       }
     }
   } else {  // PATCHED_BASE (decoder_new.h:1094-1183)
@@ -163,26 +186,35 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
     const uint32_t pll = fourth & 0x1f;
     const uint64_t pmask = pw >= 32 ? 0xffffffffull : ((1ull << pw) - 1);  // uint32 mask
     const uint32_t cfb = closest_fixed(pw + pgw);
-    const T base = be_bytes<W>(sbuf, p + 4, bw);
+    // base: bw big-endian bytes at p + 4 (inside dwords p/4 + 1 .. p/4 + 3)
+    const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)hw, 2);
+    const uint32_t w3 = (uint32_t)__builtin_amdgcn_readlane((int)hw, 3);
+    const uint64_t y = (uint64_t)(uint32_t)((((uint64_t)w2 << 32) | w1) >> sh) |
+                       ((uint64_t)(uint32_t)((((uint64_t)w3 << 32) | w2) >> sh) << 32);
+    const T base = (T)(__builtin_bswap64(y) >> (64 - 8 * bw)) & mask;
     const uint32_t vb0 = (p + 4 + bw) * 8;
     const uint32_t pb0 = (p + 4 + bw + (len8 * fbw + 7) / 8) * 8;
     // lane j < pll: entry j -> running gap (wave scan) and its OR-term
-    const uint64_t ent = lane < pll ? bits64(sbuf, pb0 + lane * cfb, cfb) : 0;
+    const uint64_t ent0 = bits64(sbuf, pb0 + (lane < pll ? lane : 0u) * cfb, cfb);
+    const uint64_t ent = lane < pll ? ent0 : 0;
     const uint32_t g = lane < pll ? (uint32_t)(pw >= 64 ? 0 : (ent >> pw)) : 0;
     const uint32_t gsum = wave_scan_dpp(g);
     const T add = (fbw >= nbits) ? (T)0 : (T)(((ent & pmask) << fbw) & (uint64_t)mask);
     if (lane < pll && gsum >= len8) *bad = 1;
-    for (uint32_t c = 0; c < len8; c += kWave) {
-      const uint32_t i = c + lane;
-      T v = i < len8 ? (field<W>(sbuf, vb0 + i * fbw, fbw) & mask) : (T)0;
+    for (uint32_t c = 0; c < len8; c += 2 * kWave) {
+      const uint32_t i0 = c + lane, i1 = c + kWave + lane;
+      T v0 = field<W>(sbuf, vb0 + (i0 < len8 ? i0 : 0u) * fbw, fbw) & mask;
+      T v1 = field<W>(sbuf, vb0 + (i1 < len8 ? i1 : 0u) * fbw, fbw) & mask;
       for (uint32_t j = 0; j < pll; j++) {  // patch entries read from their lanes
         const uint32_t gj = (uint32_t)__builtin_amdgcn_readlane((int)gsum, (int)j);
         T aj = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)add, (int)j);
         if (W > 4)
           aj |= (T)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)add >> 32), (int)j) << 32);
-        if (gj == i) v |= aj;
+        if (gj == i0) v0 |= aj;
+        if (gj == i1) v1 |= aj;
       }
-      if (i < len8) store_w<W>(dst, o + i, (T)(v + base) & mask);
+      if (i0 < len8) store_w<W>(dst, o + i0, (T)(v0 + base) & mask);
+      if (i1 < len8) store_w<W>(dst, o + i1, (T)(v1 + base) & mask);
     }
   }
 }

@@ -324,23 +324,40 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
   return nd;
 }
 
-// Expands the blocks of a batch (lane j holds block j's dpos/dout, j < nd)
-// that `sel` selects, for a column of W-byte elements. All 64 lanes of the
-// wave must call it.
+// Writes the SHORT_REPEAT blocks of a batch (lane j holds block j's
+// dpos/dout, j < nd) that `sel` selects, for a column of W-byte elements, and
+// returns the mask of the batch's other blocks (every lane < nd looks at its
+// block). All 64 lanes of the wave must call it.
+template <uint32_t W>
+__device__ __forceinline__ uint64_t expand_sr(const uint8_t *sbuf, uint32_t nd, uint64_t sel,
+                                              uint32_t dpos, uint32_t dout, uint8_t *dst,
+                                              uint32_t lane) {
+  const bool in = lane < nd;
+  const bool mine = in && ((sel >> lane) & 1);
+  // a SHORT_REPEAT block (header + at most 8 value bytes) lies inside the
+  // three dwords from its header's dword: one read for header and value
+  const uint32_t *sdw = reinterpret_cast<const uint32_t *>(sbuf) + ((in ? dpos : 0u) >> 2);
+  const uint32_t d0 = sdw[0], d1 = sdw[1], d2 = sdw[2];
+  const uint32_t k = (dpos & 3) + 1;  // first value byte, from the dword start
+  const uint32_t first = in ? (d0 >> (8 * (dpos & 3))) & 0xff : 0u;
+  if (mine && first < 64) {  // SHORT_REPEAT: big-endian value, 3..10 copies (decoder_new.h:1003-1015)
+    const uint32_t nb = ((first >> 3) & 7) + 1;
+    const uint32_t c = (first & 7) + 3;
+    const uint32_t lo = k == 4 ? d1 : __builtin_amdgcn_alignbyte(d1, d0, k);
+    const uint32_t hi = k == 4 ? d2 : __builtin_amdgcn_alignbyte(d2, d1, k);
+    const auto v = (typename Col<W>::T)(__builtin_bswap64(((uint64_t)hi << 32) | lo) >> (64 - 8 * nb)) &
+                   Col<W>::kMask;
+    for (uint32_t q = 0; q < c; q++) store_w<W>(dst, dout + q, v);
+  }
+  return __ballot(in && first >= 64);
+}
+
+// Expands the blocks of a batch that `sel` selects.
 template <uint32_t W>
 __device__ __forceinline__ void expand_batch(const uint8_t *sbuf, uint32_t nd, uint64_t sel,
                                              uint32_t dpos, uint32_t dout, uint8_t *dst,
                                              uint32_t lane, uint32_t *bad) {
-  const bool mine = lane < nd && ((sel >> lane) & 1);
-  const uint32_t myfirst = mine ? sbuf[dpos] : 0u;
-  const bool is_sr = mine && (myfirst >> 6) == 0;
-  if (is_sr) {  // SHORT_REPEAT: big-endian value, 3..10 copies (decoder_new.h:1003-1015)
-    const uint32_t nb = ((myfirst >> 3) & 7) + 1;
-    const uint32_t c = (myfirst & 7) + 3;
-    const auto v = be_bytes<W>(sbuf, dpos + 1, nb);
-    for (uint32_t k = 0; k < c; k++) store_w<W>(dst, dout + k, v);
-  }
-  uint64_t big = __ballot(mine && !is_sr);
+  uint64_t big = expand_sr<W>(sbuf, nd, sel, dpos, dout, dst, lane) & sel;
   while (big) {
     const uint32_t j = (uint32_t)__builtin_ctzll(big);
     big &= big - 1;
@@ -418,24 +435,24 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
 
 // ---------------------------------------------------------------- walker + expanders
 
-constexpr uint32_t kBatchLast = 1u << 30;     // no batch follows this one
-constexpr uint32_t kBatchRestage = 1u << 29;  // the window is restaged after this batch
+constexpr uint32_t kBatchLast = 1u << 30;  // no batch follows this one
+constexpr uint32_t kBatchWin = 1u << 28;   // the batch's blocks lie in window 1
 
 template <uint32_t kWin> struct PcLds {
-  uint8_t sbuf[kWin + 288];
+  uint8_t sbuf[2][kWin + 288];  // two stream windows
   uint32_t dpos[2][kWave];
   uint32_t dout[2][kWave];
-  uint32_t info[2];  // nd | kBatchLast | kBatchRestage
+  uint32_t info[2];  // nd | kBatchLast | kBatchWin
   uint32_t bad;
 };
 
 // Barrier protocol (every wave executes the same sequence): for batch k the
-// walker fills buffer k&1 and all waves meet at barrier B_k; the expanders
-// then write batch k while the walker walks batch k+1. A batch that ends
-// with a restage adds barrier R_k after which the walker (alone) reloads the
-// window; batch k+1 is published at B_{k+1}. Buffer k&1 is rewritten only
-// after B_{k+1}, by which time the expanders have finished batch k-1... the
-// two buffers alternate, so the walker never overwrites a batch in use.
+// walker fills descriptor buffer k&1 and all waves meet at barrier B_k; the
+// expanders then write batch k while the walker walks batch k+1 into buffer
+// (k+1)&1, which the expanders finished reading (batch k-1) before B_k. When
+// the walk needs a new window, the walker stages it right after B_k into the
+// window batch k does not use: the batches that used that window were all
+// expanded before B_k. No wave ever waits for a restage.
 template <uint32_t kWin, uint32_t kExp, uint32_t W>
 __global__ void __launch_bounds__(kWave *(1 + kExp))
 rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
@@ -463,8 +480,9 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
     s.len = (uint32_t)len;
     s.cap = cap;
     s.o = 0;
-    walk_stage<kWin>(s, L.sbuf, 0, lane);
-    walk_mirror(s, L.sbuf, lane);  // the walker's own LDS writes precede its reads
+    uint32_t wi = 0;  // window the walk is in
+    walk_stage<kWin>(s, L.sbuf[0], 0, lane);
+    walk_mirror(s, L.sbuf[0], lane);  // the walker's own LDS writes precede its reads
     int32_t st = kOk;
     bool done = (len == 0);
     for (uint32_t k = 0;; k++) {
@@ -472,7 +490,7 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
       bool restage = false;
       if (!done && st == kOk) {
         const uint32_t p0 = s.p;
-        nd = walk_batch(s, L.sbuf, lane, L.dpos[k & 1], L.dout[k & 1]);
+        nd = walk_batch(s, L.sbuf[wi], lane, L.dpos[k & 1], L.dout[k & 1]);
         done = s.stop == kStopDone;
         restage = s.stop == kStopRestage;
         st = s.err;
@@ -480,7 +498,7 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
       }
       const bool last = done || st != kOk;
       if (last) restage = false;
-      if (lane == 0) L.info[k & 1] = nd | (last ? kBatchLast : 0u) | (restage ? kBatchRestage : 0u);
+      if (lane == 0) L.info[k & 1] = nd | (last ? kBatchLast : 0u) | (wi ? kBatchWin : 0u);
       __syncthreads();  // B_k
       if (last) {
         __syncthreads();  // F: expanders are done
@@ -492,9 +510,10 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
         return;
       }
       if (restage) {
-        __syncthreads();  // R_k: the expanders have finished batch k
-        walk_stage<kWin>(s, L.sbuf, s.wpos + (s.p - s.lead), lane);
-        walk_mirror(s, L.sbuf, lane);
+        const uint32_t pos = s.wpos + (s.p - s.lead);
+        wi ^= 1;
+        walk_stage<kWin>(s, L.sbuf[wi], pos, lane);
+        walk_mirror(s, L.sbuf[wi], lane);
       }
     }
   } else {  // ---- expanders: wave e takes blocks j with j % kExp == e - 1
@@ -505,13 +524,13 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
       __syncthreads();  // B_k
       const uint32_t info = L.info[k & 1];
       const uint32_t nd = info & 0xffff;
-      expand_batch<W>(L.sbuf, nd, sel, L.dpos[k & 1][lane], L.dout[k & 1][lane], dst, lane, &bad);
+      expand_batch<W>(L.sbuf[(info & kBatchWin) ? 1 : 0], nd, sel, L.dpos[k & 1][lane],
+                      L.dout[k & 1][lane], dst, lane, &bad);
       if (info & kBatchLast) {
         if (__ballot(bad)) L.bad = 1;
         __syncthreads();  // F
         return;
       }
-      if (info & kBatchRestage) __syncthreads();  // R_k
     }
   }
 }
