@@ -105,52 +105,45 @@ enum : uint32_t { kStopNone = 0, kStopDone = 1, kStopRestage = 2 };
 constexpr uint32_t kSpan = kWave - 1;  // header positions one span step examines
 
 // Byte size and value count of the block whose header would start at LDS
-// offset q (decoder_new.h:1003-1183 read the same header fields), and whether
-// the span step may take it: the block ends inside the window (q + size <=
-// P) and is none of the cases left to the one-block path (a PATCHED_BASE
-// width sum past 64; a DELTA whose two varints do not end within the 14
-// bytes after its header, or one of them longer than 10 bytes). Bytes past P
-// never make a position acceptable: any field they feed puts the block's end
-// past P.
-__device__ __forceinline__ void span_block(const uint8_t *sbuf, uint32_t q, uint32_t P,
-                                           uint32_t &size, uint32_t &cnt, bool &ok) {
-  const uint32_t *d = reinterpret_cast<const uint32_t *>(sbuf) + (q >> 2);
-  const uint32_t d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
-  const uint32_t x0 = __builtin_amdgcn_alignbyte(d1, d0, q & 3);  // bytes q .. q+3
-  const uint32_t x1 = __builtin_amdgcn_alignbyte(d2, d1, q & 3);  // q+4 .. q+7
-  const uint32_t x2 = __builtin_amdgcn_alignbyte(d3, d2, q & 3);  // q+8 .. q+11
-  const uint32_t x3 = __builtin_amdgcn_alignbyte(d4, d3, q & 3);  // q+12 .. q+15
+// offset q = p + lane (decoder_new.h:1003-1183 read the same header fields),
+// and whether the span step may take it: the block ends inside the window
+// (q + size <= P) and is none of the cases left to the one-block path (a
+// PATCHED_BASE width sum past 64; a DELTA whose varints are not both found in
+// the 64 bytes after its header, or one of them longer than 10 bytes). Bytes
+// past P never make a position acceptable: any field they feed puts the
+// block's end past P. Called by all 64 lanes (the varint search is a ballot
+// over the span's bytes).
+__device__ __forceinline__ void span_block(const uint8_t *sbuf, uint32_t p, uint32_t lane,
+                                           uint32_t P, uint32_t &size, uint32_t &cnt, bool &ok) {
+  const uint8_t *h = sbuf + p + lane;
+  const uint32_t first = h[0], second = h[1], third = h[2], fourth = h[3];
+  const uint32_t far = h[64];
+  // varint terminators (bytes below 0x80) among bytes p .. p+127; lane l
+  // looks at bytes q+2 .. q+65
+  const uint64_t t0 = __ballot(first < 0x80), t1 = __ballot(far < 0x80);
+  const uint32_t vs = lane + 2;  // 2 .. 65
+  const uint64_t m = vs < 64 ? (t0 >> vs) | (t1 << (64 - vs)) : (t1 >> (vs - 64));
   // select forms of decode_width / closest_fixed: per-lane selects, no
   // divergent branches
   auto width = [](uint32_t k) { return k < 24 ? k + 1 : (k < 28 ? 2 * k - 22 : 8 * k - 184); };
   auto closest = [](uint32_t b) {  // b >= 2 here
     return b <= 24 ? b : (b <= 32 ? (b + 1) & ~1u : (b + 7) & ~7u);
   };
-  const uint32_t first = x0 & 0xff;
   const uint32_t kind = first >> 6;
   const uint32_t code = (first >> 1) & 0x1f;
   const uint32_t fbw = width(code);
   // the 9-bit length field plus one, kept in 8 bits (decoder_new.h:1018)
-  const uint32_t len8 = ((x0 >> 8) + 1) & 0xff;
+  const uint32_t len8 = (second + 1) & 0xff;
   const uint32_t body = (len8 * fbw + 7) >> 3;
   // PATCHED_BASE: base width, patch width, gap width, patch list length
-  const uint32_t third = (x0 >> 16) & 0xff, fourth = x0 >> 24;
   const uint32_t pw = width(third & 0x1f), pgw = (fourth >> 5) + 1;
   const uint32_t pb_size =
       5 + (third >> 5) + body + (((fourth & 0x1f) * closest(pw + pgw) + 7) >> 3);
-  // DELTA: the varints of base and delta base from byte q+2, looked for in
-  // the 14 bytes q+2 .. q+15 (tl: bytes 0..7 of that string, th: bytes
-  // 8..13); e1 / e2 index the bytes that end them
-  const uint64_t vl = ((uint64_t)x2 << 48) | ((uint64_t)x1 << 16) | (x0 >> 16);
-  const uint64_t vh = ((uint64_t)x3 << 16) | (x2 >> 16);
-  const uint64_t tl = ~vl & 0x8080808080808080ull;
-  const uint64_t th = ~vh & 0x0000808080808080ull;
-  const uint32_t e1 = tl ? (uint32_t)__builtin_ctzll(tl) >> 3
-                         : (th ? 8 + ((uint32_t)__builtin_ctzll(th) >> 3) : 14u);
-  const uint64_t tl2 = e1 < 7 ? tl & (~0ull << (8 * e1 + 8)) : 0ull;
-  const uint64_t th2 = e1 < 8 ? th : (e1 < 13 ? th & (~0ull << (8 * e1 - 56)) : 0ull);
-  const uint32_t e2 = tl2 ? (uint32_t)__builtin_ctzll(tl2) >> 3
-                          : (th2 ? 8 + ((uint32_t)__builtin_ctzll(th2) >> 3) : 14u);
+  // DELTA: the varints of base and delta base end at the first two
+  // terminators from byte q+2 (e1, e2: their indexes from q+2)
+  const uint32_t e1 = m ? (uint32_t)__builtin_ctzll(m) : 64u;
+  const uint64_t m2 = m & (m - 1);
+  const uint32_t e2 = m2 ? (uint32_t)__builtin_ctzll(m2) : 64u;
   const uint32_t nd8 = (len8 - 2) & 0xff;
   const uint32_t delta_size = 3 + e2 + (code ? (nd8 * fbw + 7) >> 3 : 0u);
   uint32_t sz = 2 + ((first >> 3) & 7), c = (first & 7) + 3;  // SHORT_REPEAT
@@ -161,10 +154,10 @@ __device__ __forceinline__ void span_block(const uint8_t *sbuf, uint32_t q, uint
   c = kind == 0 ? c : (kind == 3 ? 2 + nd8 : len8);
   good = kind == 2 ? pw + pgw <= 64 : good;
   // a varint over 10 bytes is the one-block path's error to report
-  good = kind == 3 ? e2 < 14 && e1 < 10 && e2 - e1 <= 10 : good;
+  good = kind == 3 ? e2 < 64 && e1 < 10 && e2 - e1 <= 10 : good;
   size = sz;
   cnt = c;
-  ok = good && q + sz <= P;
+  ok = good && p + lane + sz <= P;
 }
 
 // Collects up to 64 blocks into the LDS descriptor arrays: block j's LDS
@@ -201,7 +194,7 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
     {  // ---- span step over the headers at p .. p + 62
       uint32_t sz, c;
       bool ok;
-      span_block(sbuf, p + lane, s.P, sz, c, ok);
+      span_block(sbuf, p, lane, s.P, sz, c, ok);
       const uint32_t nx = lane + sz;  // next header, relative to p
       const uint32_t tab = (lane < kSpan && ok) ? (nx < kSpan ? nx : kSpan) : lane;
       // tab rises strictly along the chain until a fixed point: four links
