@@ -83,6 +83,11 @@ bool pc_shape(const codag_tune_params &t) {
          (t.threads_per_group == 128 || t.threads_per_group == 256);
 }
 
+// RLE v2 also has the walker + 2 expander waves shape.
+bool v2_pc_shape(const codag_tune_params &t) {
+  return pc_shape(t) || (t.queue_depth == 2 && t.window_bytes == 4096 && t.threads_per_group == 192);
+}
+
 // RLE v1 pipeline launch: 8 waves per chunk, 2 KiB windows in a ring of 4.
 bool pipe_shape(const codag_tune_params &t) {
   return t.threads_per_group == 512 && t.window_bytes == 2048 && t.queue_depth == 4;
@@ -229,7 +234,7 @@ codag_status codag_rlev2_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t 
   if (n_chunks > 0x7fffffffull) return fail(CODAG_ERR_INVALID, "rlev2_decode: too many chunks");
   DeviceGuard g(ctx->device);
   const codag_tune_params &t = ctx->tune[CODAG_RLEV2];
-  if (pc_shape(t))  // walker wave + expander waves per chunk
+  if (v2_pc_shape(t))  // walker wave + expander waves per chunk
     HIP_TRY(launch_rlev2_pc(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
                             d_out, d_status, t.threads_per_group / 64 - 1, pick(ctx, stream)));
   else if (t.threads_per_group == 64)  // one self-contained wave per chunk
@@ -471,6 +476,7 @@ codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_
                            p->queue_depth == 0 &&
                            (p->window_bytes == 4096 || p->window_bytes == 8192)) ||
                           (codec != CODAG_DEFLATE && pc_shape(*p)) ||
+                          (codec == CODAG_RLEV2 && v2_pc_shape(*p)) ||
                           (codec == CODAG_RLEV1 && pipe_shape(*p));
   if (codec != CODAG_DEFLATE && !wave_shape &&
       !rle_shape_supported(p->threads_per_group, p->window_bytes))
@@ -497,6 +503,7 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
     if (codec == CODAG_RLEV1) cands.push_back({512, 2048, 0, 4});  // pipelined walker + helpers
     cands.push_back({256, 4096, 0, 2});  // walker + expander shapes
     cands.push_back({128, 4096, 0, 2});
+    if (codec == CODAG_RLEV2) cands.push_back({192, 4096, 0, 2});
     if (codec == CODAG_RLEV2) {  // one-wave-per-chunk shapes
       cands.push_back({64, 4096, 0, 0});
       cands.push_back({64, 8192, 0, 0});

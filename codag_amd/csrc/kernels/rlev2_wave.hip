@@ -587,6 +587,8 @@ hipError_t launch_rlev2_pc(const uint8_t *comp, uint64_t comp_bytes, const uint6
 #define ARGS comp, comp_bytes, blk_off, col_len, n_chunks, chunk_bytes, out, status, stream
   if (nexp == 1) {
     CODAG_BY_WIDTH(pc_launch, 1)
+  } else if (nexp == 2) {
+    CODAG_BY_WIDTH(pc_launch, 2)
   } else if (nexp == 3) {
     CODAG_BY_WIDTH(pc_launch, 3)
   } else {

@@ -342,7 +342,7 @@ def test_v2_error_status_matches_oracle(ctx, k):
 
 
 @pytest.mark.parametrize("shape", [(64, 4096, 0), (64, 8192, 0), (256, 4096, 0), (128, 4096, 0),
-                                   (512, 8192, 0), (128, 4096, 2), (256, 4096, 2)])
+                                   (512, 8192, 0), (128, 4096, 2), (192, 4096, 2), (256, 4096, 2)])
 def test_v2_every_launch_shape(ctx, shape):
     say(f"rlev2 launch shape {shape}")
     raw, comp, blk, cl, n = ca.synth_rlev2(12 * CH // 8 + 5, 8, CH, seed=77)
@@ -355,7 +355,7 @@ def test_v2_every_launch_shape(ctx, shape):
         ctx.tune_set(ca.RLEV2, saved)
 
 
-@pytest.mark.parametrize("shape", [(128, 4096, 2), (256, 4096, 2), (64, 4096, 0)])
+@pytest.mark.parametrize("shape", [(128, 4096, 2), (192, 4096, 2), (256, 4096, 2), (64, 4096, 0)])
 def test_v2_walker_and_expander_kernels(ctx, shape):
     # the one-wave kernel and the walker + expander kernels on the same cases:
     # synthetic columns of several widths (many windows and batches per
