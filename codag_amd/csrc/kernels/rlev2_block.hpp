@@ -23,17 +23,12 @@ using namespace codag::dev;
 // code needs no memory load: codes 0..23 -> 1..24, 24..27 -> 26..32 step 2,
 // 28..31 -> 40..64 step 8.
 __device__ __forceinline__ uint32_t decode_width(uint32_t code) {
-  if (code < 24) return code + 1;
-  if (code < 28) return 26 + 2 * (code - 24);
-  return 40 + 8 * (code - 28);
+  return code < 24 ? code + 1 : (code < 28 ? 2 * code - 22 : 8 * code - 184);  // selects, no branches
 }
 
 // include/rlev2/utils.h:42-48 (__CLOSEST_FIXED_BIT_MAP)
 __device__ __forceinline__ uint32_t closest_fixed(uint32_t bits) {
-  if (bits <= 1) return 1;
-  if (bits <= 24) return bits;
-  if (bits <= 32) return (bits + 1) & ~1u;
-  return (bits + 7) & ~7u;
+  return bits <= 1 ? 1u : (bits <= 24 ? bits : (bits <= 32 ? (bits + 1) & ~1u : (bits + 7) & ~7u));
 }
 
 // MSB-first bit field of width f (1..32) starting at bit b of the LDS byte
@@ -118,8 +113,8 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
     const uint32_t b0 = (p + 2) * 8;
     for (uint32_t c = 0; c < len8; c += 2 * kWave) {
       const uint32_t i0 = c + lane, i1 = c + kWave + lane;
-      const T v0 = field<W>(sbuf, b0 + (i0 < len8 ? i0 : 0u) * fbw, fbw) & mask;
-      const T v1 = field<W>(sbuf, b0 + (i1 < len8 ? i1 : 0u) * fbw, fbw) & mask;
+      const T v0 = field<W>(sbuf, b0 + __umul24(i0 < len8 ? i0 : 0u, fbw), fbw) & mask;
+      const T v1 = field<W>(sbuf, b0 + __umul24(i1 < len8 ? i1 : 0u, fbw), fbw) & mask;
       if (i0 < len8) store_w<W>(dst, o + i0, v0);
       if (i1 < len8) store_w<W>(dst, o + i1, v1);
     }
@@ -172,8 +167,8 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
       };
       for (uint32_t c = 0; c < nd8; c += 2 * kWave) {  // uniform loop: all lanes scan
         const uint32_t i0 = c + lane, i1 = c + kWave + lane;
-        const T r0 = field<W>(sbuf, b0 + (i0 < nd8 ? i0 : 0u) * fbw, fbw) & mask;
-        const T r1 = field<W>(sbuf, b0 + (i1 < nd8 ? i1 : 0u) * fbw, fbw) & mask;
+        const T r0 = field<W>(sbuf, b0 + __umul24(i0 < nd8 ? i0 : 0u, fbw), fbw) & mask;
+        const T r1 = field<W>(sbuf, b0 + __umul24(i1 < nd8 ? i1 : 0u, fbw), fbw) & mask;
         step(i0, i0 < nd8 ? r0 : (T)0);
         if (c + kWave < nd8) step(i1, i1 < nd8 ? r1 : (T)0);
       }
@@ -195,7 +190,7 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
     const uint32_t vb0 = (p + 4 + bw) * 8;
     const uint32_t pb0 = (p + 4 + bw + (len8 * fbw + 7) / 8) * 8;
     // lane j < pll: entry j -> running gap (wave scan) and its OR-term
-    const uint64_t ent0 = bits64(sbuf, pb0 + (lane < pll ? lane : 0u) * cfb, cfb);
+    const uint64_t ent0 = bits64(sbuf, pb0 + __umul24(lane < pll ? lane : 0u, cfb), cfb);
     const uint64_t ent = lane < pll ? ent0 : 0;
     const uint32_t g = lane < pll ? (uint32_t)(pw >= 64 ? 0 : (ent >> pw)) : 0;
     const uint32_t gsum = wave_scan_dpp(g);
@@ -203,8 +198,8 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
     if (lane < pll && gsum >= len8) *bad = 1;
     for (uint32_t c = 0; c < len8; c += 2 * kWave) {
       const uint32_t i0 = c + lane, i1 = c + kWave + lane;
-      T v0 = field<W>(sbuf, vb0 + (i0 < len8 ? i0 : 0u) * fbw, fbw) & mask;
-      T v1 = field<W>(sbuf, vb0 + (i1 < len8 ? i1 : 0u) * fbw, fbw) & mask;
+      T v0 = field<W>(sbuf, vb0 + __umul24(i0 < len8 ? i0 : 0u, fbw), fbw) & mask;
+      T v1 = field<W>(sbuf, vb0 + __umul24(i1 < len8 ? i1 : 0u, fbw), fbw) & mask;
       for (uint32_t j = 0; j < pll; j++) {  // patch entries read from their lanes
         const uint32_t gj = (uint32_t)__builtin_amdgcn_readlane((int)gsum, (int)j);
         T aj = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)add, (int)j);
