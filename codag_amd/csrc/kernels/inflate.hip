@@ -819,6 +819,15 @@ inflate_batch_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
     i = __builtin_amdgcn_readfirstlane(__shfl(i, 0, kWave));
     if ((uint64_t)i >= n) break;
     const codag_stream_desc d = descs[i];
+    // A stream decodes serially on one wave, and the waves of a CU share its
+    // scalar pipe: in an uneven batch the largest streams set the finish
+    // time. Their waves issue first (wave priority by output size), the
+    // small streams fill the remaining issue slots.
+    const uint64_t big = d.dst_cap;
+    if (big >= (2u << 20)) __builtin_amdgcn_s_setprio(3);
+    else if (big >= (1u << 20)) __builtin_amdgcn_s_setprio(2);
+    else if (big >= (512u << 10)) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
     const Result r = inflate_stream(d.src, d.src_bytes, d.dst, d.dst_cap, flags, S);
     if (lane_id() == 0) {
       results[i].bytes_written = r.bytes;
