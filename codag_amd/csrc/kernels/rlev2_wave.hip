@@ -160,6 +160,20 @@ __device__ __forceinline__ void span_block(const uint8_t *sbuf, uint32_t p, uint
   ok = good && p + lane + sz <= P;
 }
 
+// Descriptor j of a batch: LDS offset and first value index in two arrays,
+// or packed into one word (offset | index << 13: offsets below 2^13 in a
+// 4 KiB window, value indexes below 2^19).
+template <bool kPacked>
+__device__ __forceinline__ void put_desc(uint32_t *ldpos, uint32_t *ldout, uint32_t j, uint32_t pos,
+                                         uint32_t out) {
+  if (kPacked) {
+    ldpos[j] = pos | (out << 13);
+  } else {
+    ldpos[j] = pos;
+    ldout[j] = out;
+  }
+}
+
 // Collects up to 64 blocks into the LDS descriptor arrays: block j's LDS
 // offset goes to dpos[j] and its first value index to dout[j]. Sets s.stop
 // to kStopDone when the stream is consumed or kStopRestage when the next
@@ -176,6 +190,7 @@ __device__ __forceinline__ void span_block(const uint8_t *sbuf, uint32_t p, uint
 // The first block that would overflow the chunk or the batch ends the step;
 // positions the step does not take go through the one-block path below,
 // which owns every error and window decision.
+template <bool kPacked = false>
 __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uint32_t lane,
                                                uint32_t *ldpos, uint32_t *ldout) {
   const uint32_t *sdw = reinterpret_cast<const uint32_t *>(sbuf);
@@ -224,8 +239,7 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
       if (cut) mark &= (1ull << __builtin_ctzll(cut)) - 1;
       if (mark) {
         if ((mark >> lane) & 1) {
-          ldpos[nd + rank] = p + lane;
-          ldout[nd + rank] = o + incl - cl;
+          put_desc<kPacked>(ldpos, ldout, nd + rank, p + lane, o + incl - cl);
         }
         const uint32_t last = 63 - (uint32_t)__builtin_clzll(mark);
         o += (uint32_t)__builtin_amdgcn_readlane(incl, last);
@@ -301,8 +315,7 @@ __device__ __forceinline__ uint32_t walk_batch(Walk &s, const uint8_t *sbuf, uin
       break;
     }
     if (lane == 0) {
-      ldpos[nd] = p;
-      ldout[nd] = o;
+      put_desc<kPacked>(ldpos, ldout, nd, p, o);
     }
     nd++;
     o += cnt;
@@ -430,28 +443,45 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
 
 constexpr uint32_t kBatchLast = 1u << 30;  // no batch follows this one
 constexpr uint32_t kBatchWin = 1u << 28;   // the batch's blocks lie in window 1
+constexpr uint32_t kQ = 4;                 // descriptor buffers in flight
+// A wait that outlasts this many s_sleep rounds (about 10^8 cycles, far past
+// any batch) ends the chunk with kCorrupt instead of spinning on.
+constexpr uint32_t kSpinLimit = 1u << 21;
 
 template <uint32_t kWin> struct PcLds {
   uint8_t sbuf[2][kWin + 288];  // two stream windows
-  uint32_t dpos[2][kWave];
-  uint32_t dout[2][kWave];
-  uint32_t info[2];  // nd | kBatchLast | kBatchWin
-  uint32_t bad;
+  uint32_t desc[kQ][kWave];     // packed descriptors (put_desc)
+  uint32_t info[kQ];            // nd | kBatchLast | kBatchWin
+  uint32_t pub;                 // batches published by the walker
+  uint32_t done[3];             // batches finished, per expander wave
+  uint32_t bad;                 // an expander found a patch outside its block
+  uint32_t stuck;               // a wait ran past kSpinLimit
 };
 
-// Barrier protocol (every wave executes the same sequence): for batch k the
-// walker fills descriptor buffer k&1 and all waves meet at barrier B_k; the
-// expanders then write batch k while the walker walks batch k+1 into buffer
-// (k+1)&1, which the expanders finished reading (batch k-1) before B_k. When
-// the walk needs a new window, the walker stages it right after B_k into the
-// window batch k does not use: the batches that used that window were all
-// expanded before B_k. No wave ever waits for a restage.
+__device__ __forceinline__ uint32_t lds_acquire(const uint32_t *a) {
+  return uni(__hip_atomic_load(a, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+__device__ __forceinline__ void lds_release(uint32_t *a, uint32_t v, uint32_t lane) {
+  if (lane == 0) __hip_atomic_store(a, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Queue protocol (no workgroup barriers after the first): the walker writes
+// batch k into descriptor buffer k % kQ and publishes it by raising `pub` to
+// k + 1 (release: the descriptors are in LDS first); expander e waits for pub
+// > k, writes the batch and raises done[e] to k + 1 once its reads of the
+// buffer and the window are over. The walker reuses buffer k % kQ only when
+// every done[] has passed k - kQ, and stages a window only when every batch
+// that used it is done, so it can run up to kQ batches ahead and the
+// per-batch differences between walking and expanding average out. Every
+// wait is bounded (kSpinLimit); a wait that runs out sets `stuck`, which ends
+// every wave's loop.
 template <uint32_t kWin, uint32_t kExp, uint32_t W>
 __global__ void __launch_bounds__(kWave *(1 + kExp))
 rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                 const uint64_t *__restrict__ blk_off, const uint64_t *__restrict__ col_len,
                 uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *__restrict__ out,
                 int32_t *__restrict__ status) {
+  static_assert(kWin <= 4096 && kExp <= 3, "packed descriptors and the done[] array");
   __shared__ PcLds<kWin> L;
   const uint64_t chunk = blockIdx.x;
   const uint32_t lane = lane_id();
@@ -464,26 +494,54 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   }
   const uint32_t cap = (uint32_t)(chunk_bytes / W);
   uint8_t *dst = out + chunk * chunk_bytes;
-  if (threadIdx.x == 0) L.bad = 0;
+  if (threadIdx.x == 0) {
+    L.pub = 0;
+    for (uint32_t e = 0; e < 3; e++) L.done[e] = 0;
+    L.bad = 0;
+    L.stuck = 0;
+  }
+  __syncthreads();
 
   if (wave == 0) {  // ---- walker: the critical path; its instructions issue first
     __builtin_amdgcn_s_setprio(3);
+    // waits until every expander has finished batch x - 1; false if stuck
+    auto wait_done = [&](uint32_t x) -> bool {
+      for (uint32_t spins = 0;; spins++) {
+        uint32_t m = lds_acquire(&L.done[0]);
+        for (uint32_t e = 1; e < kExp; e++) {
+          const uint32_t d = lds_acquire(&L.done[e]);
+          m = d < m ? d : m;
+        }
+        if (m >= x) return true;
+        if (spins >= kSpinLimit || lds_acquire(&L.stuck)) {
+          lds_release(&L.stuck, 1, lane);
+          return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    };
     Walk s;
     s.src = comp + boff;
     s.len = (uint32_t)len;
     s.cap = cap;
     s.o = 0;
-    uint32_t wi = 0;  // window the walk is in
+    uint32_t wi = 0;                       // window the walk is in
+    uint32_t used[2] = {0u, 0u};           // 1 + last batch that used each window (0: none)
     walk_stage<kWin>(s, L.sbuf[0], 0, lane);
     walk_mirror(s, L.sbuf[0], lane);  // the walker's own LDS writes precede its reads
     int32_t st = kOk;
     bool done = (len == 0);
     for (uint32_t k = 0;; k++) {
+      const uint32_t b = k % kQ;
+      if (k >= kQ && !wait_done(k - kQ + 1)) {
+        st = kCorrupt;
+        break;
+      }
       uint32_t nd = 0;
       bool restage = false;
       if (!done && st == kOk) {
         const uint32_t p0 = s.p;
-        nd = walk_batch(s, L.sbuf[wi], lane, L.dpos[k & 1], L.dout[k & 1]);
+        nd = walk_batch<true>(s, L.sbuf[wi], lane, L.desc[b], nullptr);
         done = s.stop == kStopDone;
         restage = s.stop == kStopRestage;
         st = s.err;
@@ -491,39 +549,57 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
       }
       const bool last = done || st != kOk;
       if (last) restage = false;
-      if (lane == 0) L.info[k & 1] = nd | (last ? kBatchLast : 0u) | (wi ? kBatchWin : 0u);
-      __syncthreads();  // B_k
+      if (wi) used[1] = k + 1;
+      else used[0] = k + 1;
+      if (lane == 0) L.info[b] = nd | (last ? kBatchLast : 0u) | (wi ? kBatchWin : 0u);
+      lds_release(&L.pub, k + 1, lane);
       if (last) {
-        __syncthreads();  // F: expanders are done
-        const uint32_t bad = L.bad;
-        if (bad && st == kOk) st = kCorrupt;
-        if (st == kOk)
-          for (uint32_t i = s.o + lane; i < cap; i += kWave) store_w<W>(dst, i, 0);
-        if (lane == 0 && status) status[chunk] = st;
-        return;
+        if (!wait_done(k + 1)) st = kCorrupt;
+        break;
       }
       if (restage) {
         const uint32_t pos = s.wpos + (s.p - s.lead);
         wi ^= 1;
+        if (!wait_done(wi ? used[1] : used[0])) {
+          st = kCorrupt;
+          break;
+        }
         walk_stage<kWin>(s, L.sbuf[wi], pos, lane);
         walk_mirror(s, L.sbuf[wi], lane);
       }
     }
+    if (lds_acquire(&L.bad) && st == kOk) st = kCorrupt;
+    if (st == kOk)
+      for (uint32_t i = s.o + lane; i < cap; i += kWave) store_w<W>(dst, i, 0);
+    if (lane == 0 && status) status[chunk] = st;
   } else {  // ---- expanders: wave e takes blocks j with j % kExp == e - 1
     uint64_t sel = 0;
     for (uint32_t j = wave - 1; j < (uint32_t)kWave; j += kExp) sel |= 1ull << j;
     uint32_t bad = 0;
     for (uint32_t k = 0;; k++) {
-      __syncthreads();  // B_k
-      const uint32_t info = L.info[k & 1];
-      const uint32_t nd = info & 0xffff;
-      expand_batch<W>(L.sbuf[(info & kBatchWin) ? 1 : 0], nd, sel, L.dpos[k & 1][lane],
-                      L.dout[k & 1][lane], dst, lane, &bad);
-      if (info & kBatchLast) {
-        if (__ballot(bad)) L.bad = 1;
-        __syncthreads();  // F
+      bool ready = false;
+      for (uint32_t spins = 0;; spins++) {
+        if (lds_acquire(&L.pub) > k) {
+          ready = true;
+          break;
+        }
+        if (spins >= kSpinLimit || lds_acquire(&L.stuck)) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (!ready) {
+        lds_release(&L.stuck, 1, lane);
         return;
       }
+      const uint32_t b = k % kQ;
+      const uint32_t info = L.info[b];
+      const uint32_t nd = info & 0xffff;
+      const uint32_t d = L.desc[b][lane];
+      expand_batch<W>(L.sbuf[(info & kBatchWin) ? 1 : 0], nd, sel, d & 0x1fff, d >> 13, dst, lane,
+                      &bad);
+      const bool any_bad = __ballot(bad) != 0;
+      if ((info & kBatchLast) && any_bad && lane == 0) L.bad = 1;
+      lds_release(&L.done[wave - 1], k + 1, lane);
+      if (info & kBatchLast) return;
     }
   }
 }
