@@ -653,11 +653,19 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
 // Decodes symbols until end-of-block. Returns a status.
 __device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, uint32_t lane) {
   for (;;) {
-    // Near the stream end a batch is one symbol, so reading past the end is
-    // caught at the symbol that does it.
-    const bool careful = bits_pos(b) + kCarefulBits > b.limit;
+    // A symbol reads at most 48 bits, so a batch of floor(left / 48) symbols
+    // cannot pass the stream end; near the end batches shrink to one symbol,
+    // and reading past the end is caught at the symbol that does it.
+    // (left * 1365) >> 16 is at most left / 48 for left below 2^16.
+    const uint32_t pos = bits_pos(b);
+    const uint32_t left = pos < b.limit ? b.limit - pos : 0u;
+    uint32_t limit = (uint32_t)kWave;
+    if (left < kCarefulBits) {
+      const uint32_t safe = (left * 1365u) >> 16;
+      limit = safe ? safe : 1u;
+    }
     uint32_t tok = 0, nt = 0;
-    const int32_t sa = decode_tokens(b, S, lane, tok, nt, careful ? 1u : (uint32_t)kWave);
+    const int32_t sa = decode_tokens(b, S, lane, tok, nt, limit);
     if (bits_past_end(b)) return kTrunc;
     const int32_t sb = out_tokens(o, lane, tok, nt);
     if (sb != kOk) return sb;
