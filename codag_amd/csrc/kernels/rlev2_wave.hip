@@ -444,9 +444,13 @@ rlev2_wave_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
 constexpr uint32_t kBatchLast = 1u << 30;  // no batch follows this one
 constexpr uint32_t kBatchWin = 1u << 28;   // the batch's blocks lie in window 1
 constexpr uint32_t kQ = 4;                 // descriptor buffers in flight
-// A wait that outlasts this many s_sleep rounds (about 10^8 cycles, far past
-// any batch) ends the chunk with kCorrupt instead of spinning on.
-constexpr uint32_t kSpinLimit = 1u << 21;
+// Waits sleep between polls: the scalar instructions of a spin are taken from
+// the other waves of the CU. The walker sleeps 2048 cycles per poll (a full
+// queue frees a buffer once per batch), an expander 1024 (a batch takes tens
+// of thousands of cycles). A wait that outlasts its poll limit (about 1.3e8
+// cycles, far past any batch) ends the chunk with kCorrupt.
+constexpr uint32_t kWalkerSleep = 32, kWalkerSpinLimit = 1u << 16;
+constexpr uint32_t kExpSleep = 16, kSpinLimit = 1u << 17;
 
 template <uint32_t kWin> struct PcLds {
   uint8_t sbuf[2][kWin + 288];  // two stream windows
@@ -455,7 +459,7 @@ template <uint32_t kWin> struct PcLds {
   uint32_t pub;                 // batches published by the walker
   uint32_t done[3];             // batches finished, per expander wave
   uint32_t bad;                 // an expander found a patch outside its block
-  uint32_t stuck;               // a wait ran past kSpinLimit
+  uint32_t stuck;               // a wait ran past its poll limit
 };
 
 __device__ __forceinline__ uint32_t lds_acquire(const uint32_t *a) {
@@ -473,7 +477,7 @@ __device__ __forceinline__ void lds_release(uint32_t *a, uint32_t v, uint32_t la
 // every done[] has passed k - kQ, and stages a window only when every batch
 // that used it is done, so it can run up to kQ batches ahead and the
 // per-batch differences between walking and expanding average out. Every
-// wait is bounded (kSpinLimit); a wait that runs out sets `stuck`, which ends
+// wait is bounded (the poll limits); a wait that runs out sets `stuck`, which ends
 // every wave's loop.
 template <uint32_t kWin, uint32_t kExp, uint32_t W>
 __global__ void __launch_bounds__(kWave *(1 + kExp))
@@ -513,11 +517,11 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
           m = d < m ? d : m;
         }
         if (m >= x) return true;
-        if (spins >= kSpinLimit || lds_acquire(&L.stuck)) {
+        if (spins >= kWalkerSpinLimit || lds_acquire(&L.stuck)) {
           lds_release(&L.stuck, 1, lane);
           return false;
         }
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(kWalkerSleep);
       }
     };
     Walk s;
@@ -584,7 +588,7 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
           break;
         }
         if (spins >= kSpinLimit || lds_acquire(&L.stuck)) break;
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(kExpSleep);
       }
       if (!ready) {
         lds_release(&L.stuck, 1, lane);
