@@ -97,7 +97,7 @@ void default_tune(codag_ctx *c) {
   // defaults: the fastest shapes on the benchmark columns (MI355X); the
   // autotuner can pick per column
   c->tune[CODAG_RLEV1] = {512, 2048, 0, 4};
-  c->tune[CODAG_RLEV2] = {256, 4096, 0, 2};
+  c->tune[CODAG_RLEV2] = {128, 4096, 0, 2};
   // 28 waves per CU: the inflate kernel's LDS and register budget allow seven per SIMD
   c->tune[CODAG_DEFLATE] = {64, 0, 28, 1};
   c->tune[0] = {0, 0, 0, 0};
