@@ -246,11 +246,17 @@ codag_status codag_deflate_compress_mixed(const uint8_t *in, uint64_t in_bytes,
 /* ---------------------------------------------------------------- autotune */
 typedef enum codag_codec { CODAG_RLEV1 = 1, CODAG_RLEV2 = 2, CODAG_DEFLATE = 3 } codag_codec;
 
+/* Launch shape of a codec's decode kernels. For RLE v1 / v2, queue_depth
+ * selects the kernel family: 0 the per-chunk kernels (RLE v2 also 64 threads
+ * = one wave per chunk, window 4096 or 8192), 2 the walker + expander kernel
+ * (window 4096; 128, 192 (RLE v2 only) or 256 threads = 1, 2 or 3 expander
+ * waves), 4 the RLE v1 pipeline (512 threads, window 2048). codag_tune_set
+ * rejects shapes that are not compiled in. */
 typedef struct codag_tune_params {
   uint32_t threads_per_group; /* workgroup size (64..1024, multiple of 64) */
   uint32_t window_bytes;      /* compressed bytes staged in LDS per window */
-  uint32_t groups_per_cu;     /* persistent launches: resident groups per CU */
-  uint32_t queue_depth;       /* persistent launches: work items claimed per grab */
+  uint32_t groups_per_cu;     /* Deflate persistent launches: resident waves per CU */
+  uint32_t queue_depth;       /* RLE: kernel family (above); Deflate: 1 */
 } codag_tune_params;
 
 codag_status codag_tune_get(codag_ctx *ctx, codag_codec codec, codag_tune_params *out);
