@@ -264,7 +264,10 @@ codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_
 /* Time each candidate on the given device-resident sample (same arguments as
  * the matching decode call; for DEFLATE d_blk_off are the n+1 offsets and
  * d_col_len is ignored) and keep the fastest whose output equals the first
- * candidate's. best_ms (nullable) gets its time. */
+ * candidate's. best_ms (nullable) gets its time. DEFLATE candidates differ in
+ * waves per CU of the persistent launch, which has min(CUs x groups_per_cu,
+ * n_chunks) waves: the sample needs more than CUs x 28 streams for them to
+ * differ at all. */
 codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_comp,
                             uint64_t comp_bytes, const uint64_t *d_blk_off, const uint64_t *d_col_len,
                             uint64_t n_chunks, uint32_t elem_bytes, uint64_t chunk_bytes,
