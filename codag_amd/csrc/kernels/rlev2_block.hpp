@@ -149,14 +149,16 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
       const bool neg = (bd == 0);  // multiplier rule, decoder_new.h:1057
       T carry = base + bd;
       const uint32_t b0 = (p + e2 + 1) * 8;  // first packed delta
-      auto step = [&](uint32_t i, T d) {     // wave prefix sum of one group
+      // 64 deltas of at most 26 bits sum below 2^32: a 32-bit scan is exact
+      const bool narrow = W <= 4 || fbw <= 26;
+      auto step = [&](uint32_t i, T d) {  // wave prefix sum of one group
         T sc;
-        if (W <= 4) sc = (T)wave_scan_dpp((uint32_t)d);
+        if (narrow) sc = (T)wave_scan_dpp((uint32_t)d);
         else sc = (T)wave_scan_dpp64((uint64_t)d);
         const T v = neg ? (T)(carry - sc) : (T)(carry + sc);
         if (i < nd8) store_w<W>(dst, o + 2 + i, v & mask);
         T tot;
-        if (W <= 4) {
+        if (narrow) {
           tot = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)sc, kWave - 1);
         } else {
           const uint64_t s64 = (uint64_t)sc;

@@ -266,6 +266,34 @@ def test_v2_every_width(ctx, code):
         assert st.tolist() == [0]
 
 
+@pytest.mark.parametrize("code", range(1, 32))
+def test_v2_delta_every_width(ctx, code):
+    # DELTA blocks with packed deltas of every width, up to 255 values (four
+    # 64-value groups), a 10-byte base varint and both signs of the delta
+    # base (zero subtracts): the 32-bit and the 64-bit prefix sums and the
+    # varint assembly, on the default kernel and the one-wave kernel
+    bits = pc.DECODE_WIDTHS[code]
+    rng = np.random.default_rng(1000 + code)
+    s = b""
+    for nvals, base, bdelta in ((255, (1 << 64) - 1, 3), (130, 1 << 63, 0), (66, 12345, -7),
+                                (3, 0, 1)):
+        deltas = [int(rng.integers(0, 1 << min(bits, 62))) << max(0, bits - 62)
+                  for _ in range(nvals - 2)]
+        deltas[0] = (1 << bits) - 1
+        s += pc.v2_delta(base, bdelta, deltas, code)
+    comp, blk, cl = single_chunk(s, pad=16)
+    saved = ctx.tune_get(ca.RLEV2)
+    try:
+        for shape in (None, (64, 4096, 0)):
+            if shape:
+                ctx.tune_set(ca.RLEV2, ca.TuneParams(shape[0], shape[1], 0, shape[2]))
+            for w in (8, 4, 2, 1):
+                st, out = _check_chunks(ctx, ca.RLEV2, comp, blk, cl, w, 512 * w)
+                assert st.tolist() == [0]
+    finally:
+        ctx.tune_set(ca.RLEV2, saved)
+
+
 @pytest.mark.parametrize("width", [8, 4, 2, 1])
 def test_v2_synth_parity(ctx, width):
     say(f"rlev2 synth width={width}")
