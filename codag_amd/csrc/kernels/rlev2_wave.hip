@@ -19,10 +19,9 @@
 //   * rlev2_wave_kernel: one wave per chunk walks and expands in turn, with no
 //     workgroup barriers; several chunks share each SIMD to hide latency.
 //   * rlev2_pc_kernel: one walker wave and kExp expander waves per chunk. The
-//     walker publishes each batch of block descriptors to an LDS double buffer
-//     and walks the next batch while the expanders write the previous one, so
-//     a chunk's time is max(walk, expand) instead of their sum and a chunk
-//     keeps more waves busy (the columns with few chunks gain most).
+//     walker publishes batches of block descriptors through a four-deep LDS
+//     queue and walks ahead while the expanders write earlier batches, so a
+//     chunk's time approaches max(walk, expand) instead of their sum.
 //
 // Semantics: include/rlev2/decoder_new.h:909-1187 (decoder_warp_orig_rdw).
 #include "kernels.hpp"
