@@ -48,6 +48,7 @@ struct codag_ctx {
   const void *g_results = nullptr;
   uint32_t g_flags = 0, g_groups = 0;
   hipStream_t g_stream = nullptr;
+  uint32_t poll_shift = 0;  // RLE v2 queue kernel: shortened waits (tests)
 };
 
 namespace {
@@ -236,7 +237,8 @@ codag_status codag_rlev2_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t 
   const codag_tune_params &t = ctx->tune[CODAG_RLEV2];
   if (v2_pc_shape(t))  // walker wave + expander waves per chunk
     HIP_TRY(launch_rlev2_pc(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
-                            d_out, d_status, t.threads_per_group / 64 - 1, pick(ctx, stream)));
+                            d_out, d_status, t.threads_per_group / 64 - 1, ctx->poll_shift,
+                            pick(ctx, stream)));
   else if (t.threads_per_group == 64)  // one self-contained wave per chunk
     HIP_TRY(launch_rlev2_wave(d_comp, comp_bytes, d_blk_off, d_col_len, n_chunks, w, chunk_bytes,
                               d_out, d_status, t.window_bytes, pick(ctx, stream)));
@@ -461,6 +463,15 @@ codag_status codag_inflate_chunks_host(codag_ctx *ctx, const uint8_t *comp, uint
 }
 
 // ------------------------------------------------------------------ tuning
+
+// Test hook, not in the public header (no counterpart in the reference):
+// shortens the bounded waits of the RLE v2 walker + expander kernel by
+// `shift` (0 restores them), so a test can drive the waits' error path.
+codag_status codag_test_poll_shift(codag_ctx *ctx, uint32_t shift) {
+  if (!ctx) return fail(CODAG_ERR_INVALID, "test_poll_shift: null ctx");
+  ctx->poll_shift = shift;
+  return CODAG_OK;
+}
 
 codag_status codag_tune_get(codag_ctx *ctx, codag_codec codec, codag_tune_params *out) {
   if (!ctx || !out || codec < CODAG_RLEV1 || codec > CODAG_DEFLATE)

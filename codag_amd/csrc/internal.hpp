@@ -8,6 +8,8 @@
 #include <cstring>
 #include <string>
 
+extern "C" codag_status codag_test_poll_shift(codag_ctx *ctx, uint32_t shift);
+
 namespace codag {
 
 // Records the message returned by codag_last_error() on this thread.

@@ -483,8 +483,11 @@ __global__ void __launch_bounds__(kWave *(1 + kExp))
 rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                 const uint64_t *__restrict__ blk_off, const uint64_t *__restrict__ col_len,
                 uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *__restrict__ out,
-                int32_t *__restrict__ status) {
+                int32_t *__restrict__ status, uint32_t poll_shift) {
   static_assert(kWin <= 4096 && kExp <= 3, "packed descriptors and the done[] array");
+  // poll limits, shortened by poll_shift (0 in use; tests end waits early)
+  const uint32_t walker_limit = poll_shift >= 31 ? 0u : kWalkerSpinLimit >> poll_shift;
+  const uint32_t exp_limit = poll_shift >= 31 ? 0u : kSpinLimit >> poll_shift;
   __shared__ PcLds<kWin> L;
   const uint64_t chunk = blockIdx.x;
   const uint32_t lane = lane_id();
@@ -516,7 +519,7 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
           m = d < m ? d : m;
         }
         if (m >= x) return true;
-        if (spins >= kWalkerSpinLimit || lds_acquire(&L.stuck)) {
+        if (spins >= walker_limit || lds_acquire(&L.stuck)) {
           lds_release(&L.stuck, 1, lane);
           return false;
         }
@@ -586,7 +589,7 @@ rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
           ready = true;
           break;
         }
-        if (spins >= kSpinLimit || lds_acquire(&L.stuck)) break;
+        if (spins >= exp_limit || lds_acquire(&L.stuck)) break;
         __builtin_amdgcn_s_sleep(kExpSleep);
       }
       if (!ready) {
@@ -623,10 +626,10 @@ void wave_launch(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_o
 template <uint32_t kExp, uint32_t W>
 void pc_launch(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
                const uint64_t *col_len, uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *out,
-               int32_t *status, hipStream_t stream) {
+               int32_t *status, uint32_t poll_shift, hipStream_t stream) {
   hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, kExp, W>), dim3((uint32_t)n_chunks),
                      dim3((1 + kExp) * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len,
-                     n_chunks, chunk_bytes, out, status);
+                     n_chunks, chunk_bytes, out, status, poll_shift);
 }
 
 // Instantiates F<..., W> for the element width w (1, 2, 4 or 8).
@@ -661,9 +664,9 @@ hipError_t launch_rlev2_wave(const uint8_t *comp, uint64_t comp_bytes, const uin
 hipError_t launch_rlev2_pc(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
                            const uint64_t *col_len, uint64_t n_chunks, uint32_t w,
                            uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t nexp,
-                           hipStream_t stream) {
+                           uint32_t poll_shift, hipStream_t stream) {
   if (n_chunks == 0) return hipSuccess;
-#define ARGS comp, comp_bytes, blk_off, col_len, n_chunks, chunk_bytes, out, status, stream
+#define ARGS comp, comp_bytes, blk_off, col_len, n_chunks, chunk_bytes, out, status, poll_shift, stream
   if (nexp == 1) {
     CODAG_BY_WIDTH(pc_launch, 1)
   } else if (nexp == 2) {

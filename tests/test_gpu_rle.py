@@ -410,6 +410,36 @@ def test_v2_walker_and_expander_kernels(ctx, shape):
         ctx.tune_set(ca.RLEV2, saved)
 
 
+def test_v2_queue_waits_are_bounded(ctx):
+    # the walker + expander kernel's waits are bounded: with the bound cut to
+    # nothing (a host-side test hook), every wait not met at once ends its
+    # chunk with CORRUPT and the launch returns; chunks that finish are still
+    # exact, and with the bound restored every chunk decodes
+    import ctypes
+    lib = ca.lib()
+    lib.codag_test_poll_shift.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    lib.codag_test_poll_shift.restype = ctypes.c_int
+    raw, comp, blk, cl, n = ca.synth_rlev2(16 * CH // 4, 4, CH, seed=77)
+    ost, oout = oracle.rlev2_decode_chunks(comp, blk[:n], cl, 4, CH, threads=8)
+    assert np.all(ost == 0)
+    saved = ctx.tune_get(ca.RLEV2)
+    say("rlev2 queue kernel with zero poll limits")
+    try:
+        for shape in ((128, 4096, 2), (256, 4096, 2)):
+            ctx.tune_set(ca.RLEV2, ca.TuneParams(shape[0], shape[1], 0, shape[2]))
+            assert lib.codag_test_poll_shift(ctx.h, 31) == 0
+            st, out = rle_decode(ctx, ca.RLEV2, comp, blk[:n], cl, 4, CH)
+            assert set(st.tolist()) <= {0, 3} and 3 in st.tolist()
+            ok = st == 0
+            assert np.array_equal(out.reshape(n, CH)[ok], oout.reshape(n, CH)[ok])
+            assert lib.codag_test_poll_shift(ctx.h, 0) == 0
+            st, out = rle_decode(ctx, ca.RLEV2, comp, blk[:n], cl, 4, CH)
+            assert np.all(st == 0) and np.array_equal(out, oout)
+    finally:
+        lib.codag_test_poll_shift(ctx.h, 0)
+        ctx.tune_set(ca.RLEV2, saved)
+
+
 def test_v2_patch_outside_block_is_corrupt(ctx):
     say("rlev2 bad patch")
     s = pc.v2_patched_base([1, 2], 0, 3, 1, [(5, 1)], 3, 3)
