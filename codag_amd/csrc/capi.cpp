@@ -48,7 +48,7 @@ struct codag_ctx {
   const void *g_results = nullptr;
   uint32_t g_flags = 0, g_groups = 0;
   hipStream_t g_stream = nullptr;
-  uint32_t poll_shift = 0;  // RLE v2 queue kernel: shortened waits (tests)
+  uint32_t poll_shift = 0;  // RLE v2 queue kernel: nonzero = waits without polls (tests)
 };
 
 namespace {
@@ -465,8 +465,9 @@ codag_status codag_inflate_chunks_host(codag_ctx *ctx, const uint8_t *comp, uint
 // ------------------------------------------------------------------ tuning
 
 // Test hook, not in the public header (no counterpart in the reference):
-// shortens the bounded waits of the RLE v2 walker + expander kernel by
-// `shift` (0 restores them), so a test can drive the waits' error path.
+// `shift` nonzero switches the RLE v2 walker + expander kernel to its
+// variant without polls in its bounded waits (0 restores the normal one), so
+// a test can drive the waits' error path.
 codag_status codag_test_poll_shift(codag_ctx *ctx, uint32_t shift) {
   if (!ctx) return fail(CODAG_ERR_INVALID, "test_poll_shift: null ctx");
   ctx->poll_shift = shift;

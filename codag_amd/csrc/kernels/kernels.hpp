@@ -54,8 +54,8 @@ hipError_t launch_rlev1_pipe(const uint8_t *comp, uint64_t comp_bytes, const uin
                              hipStream_t stream);
 
 // RLE v2, one walker wave and nexp (1, 2 or 3) expander waves per chunk.
-// poll_shift shortens the kernel's bounded waits (0 in use; a test sets 31
-// so that every wait not met at once ends its chunk with kCorrupt).
+// poll_shift nonzero (a test) launches the variant whose bounded waits have
+// no polls left, so every wait not met at once ends its chunk with kCorrupt.
 hipError_t launch_rlev2_pc(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
                            const uint64_t *col_len, uint64_t n_chunks, uint32_t w,
                            uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t nexp,

@@ -478,16 +478,18 @@ __device__ __forceinline__ void lds_release(uint32_t *a, uint32_t v, uint32_t la
 // per-batch differences between walking and expanding average out. Every
 // wait is bounded (the poll limits); a wait that runs out sets `stuck`, which ends
 // every wave's loop.
-template <uint32_t kWin, uint32_t kExp, uint32_t W>
+template <uint32_t kWin, uint32_t kExp, uint32_t W, bool kNoWait>
 __global__ void __launch_bounds__(kWave *(1 + kExp))
 rlev2_pc_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                 const uint64_t *__restrict__ blk_off, const uint64_t *__restrict__ col_len,
                 uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *__restrict__ out,
-                int32_t *__restrict__ status, uint32_t poll_shift) {
+                int32_t *__restrict__ status) {
   static_assert(kWin <= 4096 && kExp <= 3, "packed descriptors and the done[] array");
-  // poll limits, shortened by poll_shift (0 in use; tests end waits early)
-  const uint32_t walker_limit = poll_shift >= 31 ? 0u : kWalkerSpinLimit >> poll_shift;
-  const uint32_t exp_limit = poll_shift >= 31 ? 0u : kSpinLimit >> poll_shift;
+  // poll limits: compile-time constants (a runtime limit costs registers and
+  // 12% on the 64-bit column); kNoWait, for a test, makes every wait not met
+  // at once run out
+  constexpr uint32_t walker_limit = kNoWait ? 0u : kWalkerSpinLimit;
+  constexpr uint32_t exp_limit = kNoWait ? 0u : kSpinLimit;
   __shared__ PcLds<kWin> L;
   const uint64_t chunk = blockIdx.x;
   const uint32_t lane = lane_id();
@@ -627,9 +629,14 @@ template <uint32_t kExp, uint32_t W>
 void pc_launch(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
                const uint64_t *col_len, uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *out,
                int32_t *status, uint32_t poll_shift, hipStream_t stream) {
-  hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, kExp, W>), dim3((uint32_t)n_chunks),
-                     dim3((1 + kExp) * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len,
-                     n_chunks, chunk_bytes, out, status, poll_shift);
+  if (poll_shift)
+    hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, kExp, W, true>), dim3((uint32_t)n_chunks),
+                       dim3((1 + kExp) * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len,
+                       n_chunks, chunk_bytes, out, status);
+  else
+    hipLaunchKernelGGL((rle2::rlev2_pc_kernel<4096, kExp, W, false>), dim3((uint32_t)n_chunks),
+                       dim3((1 + kExp) * kWaveSize), 0, stream, comp, comp_bytes, blk_off, col_len,
+                       n_chunks, chunk_bytes, out, status);
 }
 
 // Instantiates F<..., W> for the element width w (1, 2, 4 or 8).
