@@ -491,7 +491,7 @@ codag_status codag_deflate_compress_chunks(const uint8_t *in, uint64_t in_bytes,
   return CODAG_OK;
 }
 
-// ------------------------------------------------------------ synthetic C1
+// ----------------------------------------------- synthetic RLE v1 column
 
 codag_status codag_synth_rlev1(uint64_t n_values, uint32_t w, uint64_t chunk_bytes,
                                uint64_t value_range, uint64_t seed, uint8_t **raw,
@@ -557,7 +557,7 @@ codag_status codag_synth_rlev1(uint64_t n_values, uint32_t w, uint64_t chunk_byt
                         nullptr);
 }
 
-// ------------------------------------------------------------ synthetic C2
+// ----------------------------------------------- synthetic RLE v2 column
 
 codag_status codag_synth_rlev2(uint64_t n_values, uint32_t w, uint64_t chunk_bytes,
                                uint64_t seed, uint8_t **raw, uint8_t **comp,
