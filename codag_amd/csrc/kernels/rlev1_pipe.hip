@@ -204,6 +204,7 @@ rlev1_pipe_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   // path, so its instructions issue ahead of the helpers'
   uint32_t r = 0, o = 0, wdone = 0;
   if (wave == 0) __builtin_amdgcn_s_setprio(3);
+  else if (wave <= 2) __builtin_amdgcn_s_setprio(2);  // the waves that also load and index
 
   phase_barrier();
   for (uint32_t k = 0;; k++) {
