@@ -112,8 +112,9 @@ struct Lds {
   uint16_t dist[1u << kDistBits];
   Huff<288> hl;
   Huff<32> hd;
-  uint8_t lens[320];
+  alignas(4) uint8_t lens[320];
 };
+static_assert(sizeof(Lds::lens) >= 4 * (kWave + 16), "lens doubles as the token slots");
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t j) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
@@ -168,6 +169,20 @@ __device__ __forceinline__ void bits_refill(Bits &b) {
 __device__ __forceinline__ void bits_drop(Bits &b, uint32_t n) {
   b.buf >>= n;
   b.cnt -= n;
+}
+
+// Moves the reader forward to bit P (P >= its position).
+__device__ __forceinline__ void bits_set(Bits &b, uint32_t P) {
+  const uint32_t nx = P >> 5;
+  if (nx - b.wbase >= 2u * kWave) {  // past both resident blocks
+    b.wbase = nx & ~(uint32_t)(kWave - 1);
+    bits_windows(b);
+  }
+  b.next = nx;
+  b.buf = 0;
+  b.cnt = 0;
+  bits_refill(b);
+  bits_drop(b, P & 31);
 }
 
 __device__ __forceinline__ uint32_t bits_pos(const Bits &b) { return b.next * 32 - b.cnt; }
@@ -572,17 +587,99 @@ __device__ __forceinline__ void load_fixed(Lds &S) {
   build_table(S.lens, 32, kTabDist, S.hd, S.dist, kDistBits);
 }
 
+// Window decode on the vector unit. Lane j decodes the token (a literal, or a
+// length code with its distance and extra bits) that would start at bit P + j
+// of the stream, from its own three stream dwords and the same LUT entries as
+// the symbol loop; a scalar chain through the token lengths then keeps the
+// real token starts. A token whose code is not in the LUT, an end-of-block
+// and an invalid code stop the chain (token length 128); the symbol loop
+// decodes that symbol. Batch limits are applied as the symbol loop applies
+// them: 64 tokens, or the match that brings the batch to kBatchBytes. Tokens
+// go to slot[rank] in LDS and back to tok (lane j <- token j); slots 64..79
+// take the writes of the lanes that start no token.
+// Returns true when the batch is full, false at a symbol for the loop.
+__device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
+                                              uint32_t &tok, uint32_t &nt, uint32_t &bb) {
+  const uint32_t P0 = bits_pos(b);
+  uint32_t P = P0;
+  bool full;
+  for (;;) {
+    // 64 bits from stream bit P + j; dwords past the stream read as zero, as
+    // in the bit reader
+    const uint32_t at = P + lane, D = at >> 5;
+    const uint32_t d0 = D < b.nw ? b.w[D] : 0u;
+    const uint32_t d1 = D + 1 < b.nw ? b.w[D + 1] : 0u;
+    const uint32_t d2 = D + 2 < b.nw ? b.w[D + 2] : 0u;
+    const uint32_t xl = __builtin_amdgcn_alignbit(d1, d0, at & 31u);
+    const uint32_t xh = __builtin_amdgcn_alignbit(d2, d1, at & 31u);
+    const uint64_t x = ((uint64_t)xh << 32) | xl;
+    const uint32_t e = S.lit[xl & kLitMask];
+    const uint32_t L = e & 15, lx = (e >> 4) & 7;
+    const uint64_t y = x >> (L + lx);
+    const uint32_t d = S.dist[(uint32_t)y & kDistMask];
+    const uint32_t dl = d & 15, dx = (d >> 4) & 15;
+    const uint32_t mlen = ((e >> 7) & 255) + 3 + ((uint32_t)(x >> L) & ((1u << lx) - 1));
+    const uint32_t dist = (((d >> 8) & 3) << dx) + 1 + ((uint32_t)(y >> dl) & ((1u << dx) - 1));
+    const bool lit = (e & kLitFlag) != 0;
+    const bool okm = L != 0 && lx < kXBad && dl != 0 && !(d & kDistBad);
+    const uint32_t tl = lit ? L : (okm ? L + lx + dl + dx : 128u);
+    const uint32_t tk = lit ? e : (mlen | (dist << 16));
+    const uint32_t ml = lit ? 0u : mlen;
+    // chain: token starts from window bit 0 while they lie in the window
+    uint64_t M = 0;
+    uint32_t j = 0;
+    do {
+      M |= 1ull << j;
+      j += rdlane(tl, j);
+    } while (j < (uint32_t)kWave);
+    uint32_t adv = j;  // bits used by the marked tokens
+    const bool stop = j >= 128u;
+    if (stop) {
+      adv = j - 128u;
+      M &= ~(1ull << adv);
+    }
+    // batch limits: the 64th token of the batch, or the match reaching
+    // kBatchBytes, ends it
+    const bool mk = (M >> lane) & 1;
+    const uint32_t rank = lanes_below(M);
+    const uint32_t incl = wave_scan_dpp(mk ? ml : 0u);
+    const uint64_t cm = __ballot(mk && (nt + rank + 1 >= (uint32_t)kWave || bb + incl >= kBatchBytes));
+    if (cm) {
+      const uint32_t last = (uint32_t)__builtin_ctzll(cm);
+      M &= (2ull << last) - 1;
+      adv = last + rdlane(tl, last);
+    } else {
+      bb += rdlane(incl, kWave - 1);
+    }
+    slot[((M >> lane) & 1) ? nt + rank : kWave + (lane & 15)] = tk;
+    const uint32_t n = (uint32_t)__builtin_popcountll(M);
+    const uint32_t got = slot[lane];
+    if (lane - nt < n) tok = got;
+    nt += n;
+    P += adv;
+    if (cm || stop) {
+      full = cm != 0;
+      break;
+    }
+  }
+  if (P != P0) bits_set(b, P);
+  return full;
+}
+
 // Symbol loop: decodes up to `limit` (<= 64) symbols of the current block
 // into tok (lane j <- token j, nt tokens). Returns kOk when the batch is full
 // (`limit` tokens, or the batch's matches reached kBatchBytes), kEob at
 // end-of-block, or the status of an invalid symbol. The tokens decoded before
 // an invalid symbol are valid and still have to be written. All stops leave
 // through one exit so the loop's uniform state stays in scalar registers.
-__device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t lane, uint32_t &tok,
-                                                 uint32_t &nt, uint32_t limit) {
+// A full batch (limit 64) is decoded by decode_window, and the loop takes
+// only the symbols that stop it.
+__device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
+                                                 uint32_t &tok, uint32_t &nt, uint32_t limit) {
   uint32_t bb = 0;
   int32_t st = kOk;
   for (;;) {
+    if (limit == (uint32_t)kWave && decode_window(b, S, slot, lane, tok, nt, bb)) break;
     bits_refill(b);  // >= 33 bits
     uint32_t e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
     if (e & kLitFlag) {  // literal: the token is the table entry
@@ -651,7 +748,7 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
 }
 
 // Decodes symbols until end-of-block. Returns a status.
-__device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, uint32_t lane) {
+__device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, uint32_t *slot, uint32_t lane) {
   for (;;) {
     // A symbol reads at most 48 bits, so a batch of floor(left / 48) symbols
     // cannot pass the stream end; near the end batches shrink to one symbol,
@@ -665,7 +762,7 @@ __device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, u
       limit = safe ? safe : 1u;
     }
     uint32_t tok = 0, nt = 0;
-    const int32_t sa = decode_tokens(b, S, lane, tok, nt, limit);
+    const int32_t sa = decode_tokens(b, S, slot, lane, tok, nt, limit);
     if (bits_past_end(b)) return kTrunc;
     const int32_t sb = out_tokens(o, lane, tok, nt);
     if (sb != kOk) return sb;
@@ -677,6 +774,9 @@ __device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, u
 __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t len, uint8_t *dst, uint64_t cap,
                                  uint32_t flags, Lds &S) {
   const uint32_t lane = lane_id();
+  // the code lengths are dead while a block's symbols are decoded: their
+  // bytes hold the window decoder's 64 token slots
+  uint32_t *slot = reinterpret_cast<uint32_t *>(S.lens);
   Result res{kOk, 0, 1};
   Bits b;
   // 32-bit bit positions bound the stream to < 2^28 bytes; outputs are
@@ -754,12 +854,12 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
         load_fixed(S);
         tables = 1;
       }
-      st = decode_block(b, o, S, lane);
+      st = decode_block(b, o, S, slot, lane);
     } else if (type == 2) {  // dynamic Huffman
       uint32_t nlen = 0;
       tables = 0;
       st = read_dynamic(b, S, &nlen);
-      if (st == kOk) st = decode_block(b, o, S, lane);
+      if (st == kOk) st = decode_block(b, o, S, slot, lane);
     } else {
       st = kCorrupt;  // zlib: "invalid block type"
     }

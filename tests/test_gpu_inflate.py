@@ -152,6 +152,71 @@ def test_random_corruption_matches_zlib(ctx):
     assert agree >= 0.9 * len(streams)
 
 
+def test_symbol_window_edges(ctx):
+    # the window decoder hands these to the symbol loop: codes longer than the
+    # 10-bit literal/length and 8-bit distance LUTs, end-of-block at every
+    # position of a window (frequent flushes), and batches cut by the 64-token
+    # and 512-byte limits at every token of a window
+    say("inflate window edges")
+    rng = np.random.default_rng(11)
+    parts = []
+    # skewed bytes: Huffman-only codes up to 15 bits
+    p = 0.5 ** np.arange(1, 257)
+    parts.append(z(rng.choice(256, CH, p=p / p.sum()).astype(np.uint8).tobytes(), 6, zlib.Z_HUFFMAN_ONLY))
+    # random far copies: many distance codes, some longer than 8 bits
+    base = rng.integers(0, 256, 4096, dtype=np.uint8).tobytes()
+    buf = bytearray(base)
+    while len(buf) < CH:
+        n = int(rng.integers(3, 259))
+        d = int(rng.integers(1, min(len(buf), 32768)))
+        buf += buf[len(buf) - d: len(buf) - d + n] if d >= n else (buf[-d:] * (n // d + 1))[:n]
+        buf += rng.integers(0, 256, int(rng.integers(0, 4)), dtype=np.uint8).tobytes()
+    parts.append(z(bytes(buf[:CH]), 9))
+    # many small blocks: end-of-block and empty stored blocks (sync flushes)
+    for step in (37, 200, 1001):
+        c = zlib.compressobj(6, zlib.DEFLATED, 15)
+        data = ca.synth_deflate_payload(CH // 4, CH, seed=step).tobytes()
+        out = b""
+        for i in range(0, len(data), step):
+            out += c.compress(data[i:i + step]) + c.flush(zlib.Z_SYNC_FLUSH if i % 3 else zlib.Z_FULL_FLUSH)
+        parts.append(out + c.flush())
+    # long matches between literals: the 512-byte batch cut at varied tokens
+    for k in range(4):
+        s = bytearray()
+        while len(s) < CH:
+            s += rng.integers(0, 256, int(rng.integers(1, 6 + 8 * k)), dtype=np.uint8).tobytes()
+            s += bytes([int(rng.integers(0, 256))]) * int(rng.integers(3, 300))
+        parts.append(z(bytes(s[:CH]), 6))
+    st = check(ctx, parts, CH)
+    assert np.all(st == 0)
+
+
+def test_corruption_inside_blocks_matches_zlib(ctx):
+    # flipped bytes anywhere in the stream, so the window decoder meets invalid
+    # and truncated codes mid-block
+    say("inflate fuzz inside blocks")
+    rng = np.random.default_rng(12)
+    data = ca.synth_deflate_payload(CH, CH, seed=12).tobytes()[:60000]
+    good = z(data, 6)
+    streams = []
+    for t in range(400):
+        s = bytearray(good)
+        for _ in range(int(rng.integers(1, 3))):
+            s[int(rng.integers(2, len(s)))] ^= int(rng.integers(1, 256))
+        streams.append(bytes(s))
+    comp, off = container(streams)
+    st, ln, out = inflate(ctx, comp, off, CH)
+    agree = 0
+    for i, s in enumerate(streams):
+        rc, ref = oracle.zlib_inflate(s, CH)
+        if rc == 0:
+            assert st[i] == 0 and out[i * CH: i * CH + len(ref)].tobytes() == ref.tobytes()
+        else:
+            assert st[i] != 0, i
+        agree += int(st[i] == rc)
+    assert agree >= 0.9 * len(streams)
+
+
 def test_chunked_container_parity(ctx):
     say("inflate container 64 chunks")
     raw = ca.synth_deflate_payload(64 * CH - 999, CH, seed=8)
