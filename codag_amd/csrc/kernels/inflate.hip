@@ -605,11 +605,13 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
   bool full;
   for (;;) {
     // 64 bits from stream bit P + j; dwords past the stream read as zero, as
-    // in the bit reader
-    const uint32_t at = P + lane, D = at >> 5;
-    const uint32_t d0 = D < b.nw ? b.w[D] : 0u;
-    const uint32_t d1 = D + 1 < b.nw ? b.w[D + 1] : 0u;
-    const uint32_t d2 = D + 2 < b.nw ? b.w[D + 2] : 0u;
+    // in the bit reader (a full batch starts >= 384 bytes before the end, so
+    // nw > 0 and the clamped loads stay in the stream)
+    const uint32_t at = P + lane, D = at >> 5, lastw = b.nw - 1;
+    uint32_t d0 = b.w[min(D, lastw)], d1 = b.w[min(D + 1, lastw)], d2 = b.w[min(D + 2, lastw)];
+    d0 = D < b.nw ? d0 : 0u;
+    d1 = D + 1 < b.nw ? d1 : 0u;
+    d2 = D + 2 < b.nw ? d2 : 0u;
     const uint32_t xl = __builtin_amdgcn_alignbit(d1, d0, at & 31u);
     const uint32_t xh = __builtin_amdgcn_alignbit(d2, d1, at & 31u);
     const uint64_t x = ((uint64_t)xh << 32) | xl;
@@ -621,38 +623,39 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     const uint32_t mlen = ((e >> 7) & 255) + 3 + ((uint32_t)(x >> L) & ((1u << lx) - 1));
     const uint32_t dist = (((d >> 8) & 3) << dx) + 1 + ((uint32_t)(y >> dl) & ((1u << dx) - 1));
     const bool lit = (e & kLitFlag) != 0;
-    const bool okm = L != 0 && lx < kXBad && dl != 0 && !(d & kDistBad);
-    const uint32_t tl = lit ? L : (okm ? L + lx + dl + dx : 128u);
+    // a length code stops the chain if it or its distance code is not in the
+    // LUT, if it is end-of-block or invalid (extra field 6, 7), or if the
+    // distance code is 30 or 31
+    const uint32_t bad = (((L - 1u) | (dl - 1u)) >> 31) | ((lx + 2u) >> 3) | ((d & kDistBad) >> 13);
+    const uint32_t tl = lit ? L : (bad ? 128u : L + lx + dl + dx);
     const uint32_t tk = lit ? e : (mlen | (dist << 16));
     const uint32_t ml = lit ? 0u : mlen;
-    // chain: token starts from window bit 0 while they lie in the window
+    // chain: token starts from window bit 0 while they lie in the window; a
+    // stop lane (length 128) ends it at j >= 128
     uint64_t M = 0;
     uint32_t j = 0;
     do {
       M |= 1ull << j;
       j += rdlane(tl, j);
     } while (j < (uint32_t)kWave);
-    uint32_t adv = j;  // bits used by the marked tokens
     const bool stop = j >= 128u;
-    if (stop) {
-      adv = j - 128u;
-      M &= ~(1ull << adv);
-    }
+    uint32_t adv = j & 127u;  // bits used by the tokens before the stop / window end
     // batch limits: the 64th token of the batch, or the match reaching
     // kBatchBytes, ends it
-    const bool mk = (M >> lane) & 1;
-    const uint32_t rank = lanes_below(M);
+    const bool mk = ((M >> lane) & 1) && tl < 128u;
+    const uint32_t rank = lanes_below(__ballot(mk));
     const uint32_t incl = wave_scan_dpp(mk ? ml : 0u);
     const uint64_t cm = __ballot(mk && (nt + rank + 1 >= (uint32_t)kWave || bb + incl >= kBatchBytes));
+    uint32_t lastc = kWave - 1;
     if (cm) {
-      const uint32_t last = (uint32_t)__builtin_ctzll(cm);
-      M &= (2ull << last) - 1;
-      adv = last + rdlane(tl, last);
+      lastc = (uint32_t)__builtin_ctzll(cm);
+      adv = lastc + rdlane(tl, lastc);
     } else {
       bb += rdlane(incl, kWave - 1);
     }
-    slot[((M >> lane) & 1) ? nt + rank : kWave + (lane & 15)] = tk;
-    const uint32_t n = (uint32_t)__builtin_popcountll(M);
+    const bool keep = mk && lane <= lastc;
+    slot[keep ? nt + rank : kWave + (lane & 15)] = tk;
+    const uint32_t n = (uint32_t)__builtin_popcountll(__ballot(keep));
     const uint32_t got = slot[lane];
     if (lane - nt < n) tok = got;
     nt += n;
