@@ -587,6 +587,37 @@ __device__ __forceinline__ void load_fixed(Lds &S) {
   build_table(S.lens, 32, kTabDist, S.hd, S.dist, kDistBits);
 }
 
+// The three stream dwords that hold the 64 bits from stream bit `at` (the
+// window decoder's per-lane input), as global loads clamped to the stream's
+// dwords; field_bits zeroes the dwords past the stream, as the bit reader
+// does, where the bits are used, so the loads stay in flight until then. A
+// full batch starts >= 384 bytes before the stream end, so nw > 0 where
+// these are called.
+struct Field {
+  uint32_t d0, d1, d2;
+};
+
+__device__ __forceinline__ Field load_field(const Bits &b, uint32_t at) {
+  typedef __attribute__((address_space(1))) const uint32_t gdword;
+  const gdword *w = (const gdword *)b.w;
+  const uint32_t D = at >> 5, lastw = b.nw - 1;
+  Field f;
+  f.d0 = w[min(D, lastw)];
+  f.d1 = w[min(D + 1, lastw)];
+  f.d2 = w[min(D + 2, lastw)];
+  return f;
+}
+
+__device__ __forceinline__ uint64_t field_bits(const Bits &b, const Field &f, uint32_t at) {
+  const uint32_t D = at >> 5;
+  const uint32_t d0 = D < b.nw ? f.d0 : 0u;
+  const uint32_t d1 = D + 1 < b.nw ? f.d1 : 0u;
+  const uint32_t d2 = D + 2 < b.nw ? f.d2 : 0u;
+  const uint32_t xl = __builtin_amdgcn_alignbit(d1, d0, at & 31u);
+  const uint32_t xh = __builtin_amdgcn_alignbit(d2, d1, at & 31u);
+  return ((uint64_t)xh << 32) | xl;
+}
+
 // Window decode on the vector unit. Lane j decodes the token (a literal, or a
 // length code with its distance and extra bits) that would start at bit P + j
 // of the stream, from its own three stream dwords and the same LUT entries as
@@ -594,27 +625,21 @@ __device__ __forceinline__ void load_fixed(Lds &S) {
 // real token starts. A token whose code is not in the LUT, an end-of-block
 // and an invalid code stop the chain (token length 128); the symbol loop
 // decodes that symbol. Batch limits are applied as the symbol loop applies
-// them: 64 tokens, or the match that brings the batch to kBatchBytes. Tokens
-// go to slot[rank] in LDS and back to tok (lane j <- token j); slots 64..79
-// take the writes of the lanes that start no token.
+// them: 64 tokens, or the match that brings the batch to kBatchBytes. Token k
+// of the batch goes to slot[k] in LDS and bit k of wm is set (the caller moves
+// those slots into tok once per batch); slots 64..79 take the writes of the
+// lanes that start no token. The next window's loads are issued as soon as
+// the chain has placed it, ahead of the batch-limit work.
 // Returns true when the batch is full, false at a symbol for the loop.
 __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
-                                              uint32_t &tok, uint32_t &nt, uint32_t &bb) {
+                                              uint64_t &wm, uint32_t &nt, uint32_t &bb) {
   const uint32_t P0 = bits_pos(b);
   uint32_t P = P0;
   bool full;
+  Field f = load_field(b, P + lane);
   for (;;) {
-    // 64 bits from stream bit P + j; dwords past the stream read as zero, as
-    // in the bit reader (a full batch starts >= 384 bytes before the end, so
-    // nw > 0 and the clamped loads stay in the stream)
-    const uint32_t at = P + lane, D = at >> 5, lastw = b.nw - 1;
-    uint32_t d0 = b.w[min(D, lastw)], d1 = b.w[min(D + 1, lastw)], d2 = b.w[min(D + 2, lastw)];
-    d0 = D < b.nw ? d0 : 0u;
-    d1 = D + 1 < b.nw ? d1 : 0u;
-    d2 = D + 2 < b.nw ? d2 : 0u;
-    const uint32_t xl = __builtin_amdgcn_alignbit(d1, d0, at & 31u);
-    const uint32_t xh = __builtin_amdgcn_alignbit(d2, d1, at & 31u);
-    const uint64_t x = ((uint64_t)xh << 32) | xl;
+    const uint64_t x = field_bits(b, f, P + lane);
+    const uint32_t xl = (uint32_t)x;
     const uint32_t e = S.lit[xl & kLitMask];
     const uint32_t L = e & 15, lx = (e >> 4) & 7;
     const uint64_t y = x >> (L + lx);
@@ -640,6 +665,8 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     } while (j < (uint32_t)kWave);
     const bool stop = j >= 128u;
     uint32_t adv = j & 127u;  // bits used by the tokens before the stop / window end
+    // the next window, unless this one stops or ends the batch
+    f = load_field(b, P + adv + lane);
     // batch limits: the 64th token of the batch, or the match reaching
     // kBatchBytes, ends it
     const bool mk = ((M >> lane) & 1) && tl < 128u;
@@ -656,8 +683,7 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     const bool keep = mk && lane <= lastc;
     slot[keep ? nt + rank : kWave + (lane & 15)] = tk;
     const uint32_t n = (uint32_t)__builtin_popcountll(__ballot(keep));
-    const uint32_t got = slot[lane];
-    if (lane - nt < n) tok = got;
+    if (n) wm |= (~0ull >> (kWave - n)) << nt;
     nt += n;
     P += adv;
     if (cm || stop) {
@@ -680,9 +706,10 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
 __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
                                                  uint32_t &tok, uint32_t &nt, uint32_t limit) {
   uint32_t bb = 0;
+  uint64_t wm = 0;  // batch tokens that are in the LDS slots
   int32_t st = kOk;
   for (;;) {
-    if (limit == (uint32_t)kWave && decode_window(b, S, slot, lane, tok, nt, bb)) break;
+    if (limit == (uint32_t)kWave && decode_window(b, S, slot, lane, wm, nt, bb)) break;
     bits_refill(b);  // >= 33 bits
     uint32_t e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
     if (e & kLitFlag) {  // literal: the token is the table entry
@@ -746,6 +773,10 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
     if (lane == nt) tok = len | (dist << 16);
     bb += len;
     if (++nt == limit || bb >= kBatchBytes) break;
+  }
+  if (wm) {
+    const uint32_t got = slot[lane];
+    if ((wm >> lane) & 1) tok = got;
   }
   return st;
 }
