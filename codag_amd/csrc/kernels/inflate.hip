@@ -339,7 +339,7 @@ __device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t to
   const uint32_t mlen = tok & 0x1ff;
   const uint32_t dist = tok >> 16;
   const uint32_t tl = lit ? 1u : (mat ? mlen : 0u);
-  const uint32_t incl = wave_incl_scan(tl);
+  const uint32_t incl = wave_scan_dpp(tl);
   const uint32_t x = o.pos + incl - tl;  // output position of this token
   uint32_t err = 0;
   if (act && x + tl > o.cap) err = kOverflow;
