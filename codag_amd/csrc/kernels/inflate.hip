@@ -353,7 +353,30 @@ __device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t to
   const bool far = mat && (x - dist) + kRing < pend;
   const uint64_t fm = __ballot(far);
   if (fm) gather_far(o, lane, fm, x, dist, mlen);
-  uint64_t nm = __ballot(mat && !far);
+  // A near match whose source ends before the batch's first byte reads only
+  // bytes that were in the ring before the batch, and writes bytes that no
+  // earlier token of the batch reads: such matches of up to 16 bytes are
+  // copied together, each lane its own, 4 bytes per step (bytes past the
+  // match's end repeat its last byte's copy). The other near matches follow
+  // in token order.
+  const bool ind = mat && !far && mlen <= 16u && (x - dist) + mlen <= o.pos;
+  if (__ballot(ind)) {
+    const uint32_t src = x - dist, last = mlen - 1;
+#pragma unroll
+    for (uint32_t i = 0; i < 16u; i += 4) {
+      if (!__ballot(ind && i < mlen)) break;
+      if (ind && i < mlen) {
+        const uint32_t k1 = min(i + 1, last), k2 = min(i + 2, last), k3 = min(i + 3, last);
+        const uint8_t b0 = o.ring[ridx(o, src + i)], b1 = o.ring[ridx(o, src + k1)];
+        const uint8_t b2 = o.ring[ridx(o, src + k2)], b3 = o.ring[ridx(o, src + k3)];
+        o.ring[ridx(o, x + i)] = b0;
+        o.ring[ridx(o, x + k1)] = b1;
+        o.ring[ridx(o, x + k2)] = b2;
+        o.ring[ridx(o, x + k3)] = b3;
+      }
+    }
+  }
+  uint64_t nm = __ballot(mat && !far && !ind);
   while (nm) {  // in token order: a match may read bytes of earlier tokens
     const uint32_t j = (uint32_t)__builtin_ctzll(nm);
     nm &= nm - 1;
