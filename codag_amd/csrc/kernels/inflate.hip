@@ -9,18 +9,23 @@
 //    lanes (10-bit literal/length table, 8-bit distance table, canonical
 //    count/symbol arrays for the rare longer codes). Table entries carry the
 //    RFC 1951 length/distance base and extra-bit count directly; a literal's
-//    entry has bit 31 set so the symbol loop tests it with one scalar compare.
+//    entry has bit 15 set so the symbol loop tests it with one scalar compare.
 //  * Two phases per batch of up to 64 symbols:
-//      1. the symbol loop (scalar unit) only decodes: one LDS table read per
-//         literal/length and per distance code, and the decoded token is
-//         parked in lane j of a VGPR (token j);
+//      1. decode only. The window decoder (vector unit) has lane j decode the
+//         token that would start at bit P + j of a 64-bit window, and a
+//         scalar readlane chain through the token lengths keeps the real
+//         token starts; the scalar symbol loop decodes the symbols that stop
+//         the chain (codes longer than the LUT, end-of-block, invalid codes)
+//         and the batches near the stream end. Token j of the batch is parked
+//         in lane j of a VGPR;
 //      2. the whole wave materialises the batch: a wave prefix sum gives each
 //         token its output offset, literals go to an LDS ring in one store,
 //         matches whose source has left the ring are gathered from HBM (eight
-//         per memory round trip), and the remaining matches are copied
-//         ring -> ring in token order, 64 bytes per step.
-//    Decoding never depends on output bytes, so the scalar loop runs without
-//    waiting on memory beyond its own table reads.
+//         per memory round trip), short matches whose source precedes the
+//         batch are copied together (one lane each), and the remaining
+//         matches are copied ring -> ring in token order, 64 bytes per step.
+//    Decoding never depends on output bytes, so it runs without waiting on
+//    memory beyond its own table reads and stream loads.
 //  * The ring is written to HBM with 16-byte stores after each batch that
 //    leaves at least kFlushMin unwritten bytes; Adler-32 partial sums (sum b,
 //    sum pos*b) are accumulated on the way, so the trailer is verified without
@@ -91,7 +96,7 @@ template <int NSYM> struct Huff {
 // Then (a) the batch never overwrites a ring byte that is not yet in HBM and
 // (b) a match whose source lies before the ring's oldest intact byte reads
 // only bytes that are already in HBM, when kRing >= T + 258 + kFlushMin.
-// Sized so a wave needs under 6 KiB of LDS (about seven waves per SIMD).
+// Sized so a wave needs under 6 KiB of LDS.
 constexpr uint32_t kRing = 2048;
 constexpr uint32_t kRingMask = kRing - 1;
 constexpr uint32_t kBatchBytes = 512;
@@ -101,9 +106,9 @@ static_assert(kRing >= kBatchBytes + 258 + 64 + 258 + kFlushMin, "ring too small
 // from the stream end cannot read past it.
 constexpr uint32_t kCarefulBits = 64 * 48;
 constexpr uint32_t kGather = 8;  // far-match pieces in flight per memory round trip
-// The serial symbol loop is latency-bound, so throughput grows with the waves
-// resident per SIMD: LDS (under 6 KiB per wave) allows about seven, and the
-// register budget is held to match.
+// The serial decode is latency-bound, so throughput grows with the waves
+// resident per SIMD: LDS (5760 B, twelve 512-byte allocation blocks per wave)
+// allows 26 per CU, and the register budget is held to seven per SIMD.
 constexpr int kWavesPerSimd = 7;
 
 struct Lds {
