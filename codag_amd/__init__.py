@@ -157,6 +157,9 @@ def lib():
         _sig(L, "codag_memset_d", st, [_vp, _vp, ctypes.c_int, _u64])
         _sig(L, "codag_copy_d2d", st, [_vp, _vp, _vp, _u64])
         _sig(L, "codag_sync", st, [_vp])
+        _sig(L, "codag_stream_create", st, [_vp, pp])
+        _sig(L, "codag_stream_destroy", st, [_vp, _vp])
+        _sig(L, "codag_stream_sync", st, [_vp, _vp])
         _sig(L, "codag_timer_start", st, [_vp])
         _sig(L, "codag_timer_stop", st, [_vp, ctypes.POINTER(ctypes.c_float)])
         _sig(L, "codag_device_info", st, [_vp, ctypes.c_char_p, _u32, ctypes.POINTER(_u32)])
@@ -236,6 +239,18 @@ class Context:
     def sync(self):
         check(lib().codag_sync(self.h), "sync")
 
+    def stream_create(self) -> ctypes.c_void_p:
+        """A new HIP stream of this context's device (codag_stream_create)."""
+        s = ctypes.c_void_p()
+        check(lib().codag_stream_create(self.h, ctypes.byref(s)), "stream_create")
+        return s
+
+    def stream_destroy(self, stream):
+        check(lib().codag_stream_destroy(self.h, stream), "stream_destroy")
+
+    def stream_sync(self, stream=None):
+        check(lib().codag_stream_sync(self.h, stream), "stream_sync")
+
     def copy(self, dst, src, nbytes: int):
         """Asynchronous device-to-device copy by the library's plain copy kernel."""
         check(lib().codag_copy_d2d(self.h, _dp(dst), _dp(src), nbytes), "copy_d2d")
@@ -277,8 +292,8 @@ class Context:
                                          _dp(out), _dp(out_len), _dp(status), flags, None),
               "inflate_chunks")
 
-    def inflate_batch(self, descs, n, results, flags=0):
-        check(lib().codag_inflate_batch(self.h, _dp(descs), n, _dp(results), flags, None),
+    def inflate_batch(self, descs, n, results, flags=0, stream=None):
+        check(lib().codag_inflate_batch(self.h, _dp(descs), n, _dp(results), flags, stream),
               "inflate_batch")
 
     def tune_get(self, codec) -> TuneParams:

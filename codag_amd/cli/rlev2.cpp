@@ -2,14 +2,66 @@
 //
 // Usage: rlev2 <raw column> <1|2|4|8>      (positional, src/rlev2/rlev2.cu:118-126)
 //    or: rlev2 -f <raw column> -t <1|2|4|8> (README.md:17)
-// Encodes with 128 KiB chunks (rlev2.cu:142), decodes on the GPU, checks the
-// result (check_diff, rlev2.cu:18-43) and prints the reference's lines
-// (rlev2.cu:145, 85, decoder_trans.h:124).
+// Behaviour follows rlev2.cu:44-115 through the reference's host API
+// (codag_compat.hpp): rlev2::compress_gpu_orig encodes with 128 KiB chunks,
+// then two groups of five rlev2::decompress_gpu_orig calls (queue sizes 48
+// and 64, NUM_CHUNKS 1..16) each print "<seconds> secs" after the group's
+// "CHUNK \t48\tencoded\tmeta\t" prefix, and every result is compared with the
+// input (check_diff, rlev2.cu:18-43: "fail at i in(out)"). Exit status: 0
+// when every decode matches, 2 on a mismatch, 1 on an error.
 #include <cstring>
 #include <string>
-#include <vector>
 
 #include "cli_common.hpp"
+#include "codag_compat.hpp"
+
+template <typename T, int QS, int NW, int NC>
+static bool variant(const T *in, uint64_t size, const uint8_t *enc, uint64_t enc_n, uint64_t n,
+                    rlev2::blk_off_t *blk, rlev2::col_len_t *col, uint64_t chunk) {
+  T *dec = nullptr;
+  uint64_t dec_n = 0;
+  rlev2::decompress_gpu_orig<1, QS, T, NW, NC>(enc, enc_n, n, blk, col, dec, dec_n, chunk);
+  if (!dec) return false;
+  bool ok = dec_n == size;
+  const uint8_t *a = reinterpret_cast<const uint8_t *>(in);
+  const uint8_t *b = reinterpret_cast<const uint8_t *>(dec);
+  const int64_t d = first_diff(a, b, dec_n < size ? dec_n : size);
+  if (d >= 0) {
+    std::printf("fail at %llu %x(%x)\n", (unsigned long long)d, a[d], b[d]);
+    ok = false;
+  }
+  delete[] dec;
+  return ok;
+}
+
+template <typename T> static int run(const T *in, uint64_t size) {
+  const uint64_t chunk = 1024 * 128;  // rlev2.cu:142
+  uint8_t *enc = nullptr;
+  uint64_t enc_n = 0, meta = 0, n = 0;
+  rlev2::blk_off_t *blk = nullptr;
+  rlev2::col_len_t *col = nullptr;
+  rlev2::compress_gpu_orig<1, T>(in, size, enc, enc_n, meta, n, blk, col, chunk);
+  if (!enc) return 1;
+  bool ok = true;
+  std::printf("%llu \t%d\t%llu\t%llu\t", (unsigned long long)chunk, 48,
+              (unsigned long long)enc_n, (unsigned long long)meta);
+  ok = variant<T, 48, 2, 1>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  ok = variant<T, 48, 2, 2>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  ok = variant<T, 48, 2, 4>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  ok = variant<T, 48, 2, 8>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  ok = variant<T, 48, 2, 16>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  std::printf("%llu \t%d\t%llu\t%llu\t", (unsigned long long)chunk, 48,
+              (unsigned long long)enc_n, (unsigned long long)meta);
+  ok = variant<T, 64, 1, 1>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  ok = variant<T, 64, 1, 2>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  ok = variant<T, 64, 1, 4>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  ok = variant<T, 64, 1, 8>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  ok = variant<T, 64, 1, 16>(in, size, enc, enc_n, n, blk, col, chunk) && ok;
+  delete[] blk;
+  delete[] col;
+  delete[] enc;
+  return ok ? 0 : 2;
+}
 
 int main(int argc, char **argv) {
   std::string fname;
@@ -24,49 +76,19 @@ int main(int argc, char **argv) {
     std::fprintf(stderr, "please provide arguments INPUT_FILE SIZE_OF_DATATYPE\n");
     return 1;
   }
-  if (width != 1 && width != 2 && width != 4 && width != 8) {
-    std::fprintf(stderr, "SIZE_OF_DATATYPE must be 1, 2, 4 or 8\n");
-    return 1;
-  }
   MappedFile in;
   if (!in.open(fname.c_str())) {
     std::printf("fatal error: input file open error\n");
     return 1;
   }
-  const uint64_t chunk = 1024 * 128;
   std::printf("CHUNK_SIZE\t BUFFER SIZE \t COMPRESSED SIZE \t META DATA SIZE \t DECOMPRESSION TIME\n");
-  uint8_t *comp = nullptr;
-  uint64_t comp_bytes = 0, *blk = nullptr, *col = nullptr, n = 0;
-  codag_status s = codag_rlev2_encode(in.bytes(), in.n, (uint32_t)width, chunk, &comp, &comp_bytes,
-                                      &blk, &col, &n);
-  if (s) die("encode", s);
-  // meta_data_size as encoder_trans.h:505 counts it
-  const uint64_t meta = 8 * n * 32 + 8 * (n + 1);
-  codag_ctx *ctx = nullptr;
-  if ((s = codag_ctx_create(0, &ctx))) die("GPU context", s);
-  std::vector<uint8_t> out(in.n ? in.n : 1);
-  uint64_t out_n = 0;
-  double ksec = 0, tsec = 0;
-  std::printf("%llu \t%d\t%llu\t%llu\t", (unsigned long long)chunk, 48,
-              (unsigned long long)comp_bytes, (unsigned long long)meta);
-  s = codag_rlev2_decode_host(ctx, comp, comp_bytes, blk, col, n, (uint32_t)width, chunk,
-                              out.data(), out.size(), &out_n, &ksec, &tsec);
-  if (s) die("decode", s);
-  std::printf("%g secs\n", ksec);
-  int rc = 0;
-  if (out_n != in.n) {
-    std::printf("fail: decoded %llu bytes, expected %llu\n", (unsigned long long)out_n,
-                (unsigned long long)in.n);
-    rc = 2;
+  switch (width) {
+  case 1: return run(reinterpret_cast<const uint8_t *>(in.bytes()), in.n);
+  case 2: return run(reinterpret_cast<const uint16_t *>(in.bytes()), in.n);
+  case 4: return run(reinterpret_cast<const uint32_t *>(in.bytes()), in.n);
+  case 8: return run(reinterpret_cast<const uint64_t *>(in.bytes()), in.n);
+  default:
+    std::fprintf(stderr, "SIZE_OF_DATATYPE must be 1, 2, 4 or 8\n");
+    return 1;
   }
-  int64_t d = first_diff(in.bytes(), out.data(), out_n < in.n ? out_n : in.n);
-  if (d >= 0) {
-    std::printf("fail at %lld %x(%x)\n", (long long)d, in.bytes()[d], out[d]);
-    rc = 2;
-  }
-  codag_host_free(comp);
-  codag_host_free(blk);
-  codag_host_free(col);
-  codag_ctx_destroy(ctx);
-  return rc;
 }

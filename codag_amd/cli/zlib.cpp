@@ -2,12 +2,14 @@
 //
 // Usage: zlib <X_comp.bin> <out> <X_blk_offset.bin> <chunk_size>   (src/zlib/zlib.cu:28-38)
 //    or: zlib <X_comp.bin> <X_blk_offset.bin> <chunk_size>         (no output file)
-// Decodes the chunked zlib container on the GPU, prints "\t<secs>" like
-// include/zlib/zlib.h:1683 and writes the decoded bytes to <out>.
+// Decodes the chunked zlib container through the reference's host API
+// (deflate::decompress_gpu, codag_compat.hpp), which prints "\t<seconds>"
+// like include/zlib/zlib.h:1683, and writes the decoded bytes to <out>.
+// Exit status: 0 on success, 1 on an error.
 #include <cstring>
-#include <vector>
 
 #include "cli_common.hpp"
+#include "codag_compat.hpp"
 
 int main(int argc, char **argv) {
   const char *comp_f = nullptr, *out_f = nullptr, *blk_f = nullptr, *cs = nullptr;
@@ -38,22 +40,17 @@ int main(int argc, char **argv) {
     std::fprintf(stderr, "Fatal Error: blk_offset file or chunk size invalid\n");
     return 1;
   }
-  const uint64_t *b = reinterpret_cast<const uint64_t *>(blk.p);
-  const uint64_t orig = b[0];
-  codag_ctx *ctx = nullptr;
-  codag_status s = codag_ctx_create(0, &ctx);
-  if (s) die("GPU context", s);
-  std::vector<uint8_t> out(orig ? orig : 1);
+  uint8_t *out = nullptr;
   uint64_t out_n = 0;
-  double ksec = 0, tsec = 0;
-  s = codag_inflate_chunks_host(ctx, comp.bytes(), comp.n, b, blk.n / 8, chunk, 0, out.data(),
-                                out.size(), &out_n, &ksec, &tsec);
-  if (s) die("decode", s);
-  std::printf("\t%g\n", ksec);
-  if (out_f && !write_file(out_f, out.data(), out_n)) {
+  deflate::decompress_gpu<uint32_t, 262144 / 2, 64, 8, 1>(
+      comp.bytes(), &out, comp.n, &out_n,
+      const_cast<uint64_t *>(reinterpret_cast<const uint64_t *>(blk.p)), blk.n, chunk);
+  if (!out) return 1;
+  int rc = 0;
+  if (out_f && !write_file(out_f, out, out_n)) {
     std::printf("Fatal Error: OUTPUT File open error\n");
-    return 1;
+    rc = 1;
   }
-  codag_ctx_destroy(ctx);
-  return 0;
+  std::free(out);
+  return rc;
 }

@@ -1,6 +1,10 @@
 // capi.cpp -- the C ABI of libcodag: contexts, decode entry points, host
 // convenience wrappers, batch launch, autotuner and device helpers.
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <chrono>
@@ -31,12 +35,28 @@ using namespace codag;
       return fail(CODAG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
   } while (0)
 
+namespace {
+constexpr int kCounterSlots = 16;
+constexpr uint32_t kCounterStride = 256;  // bytes between two slots' counters
+}  // namespace
+
 struct codag_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evs[256] = {};
-  uint32_t *d_counter = nullptr;  // persistent-batch work counter
+  // Work-queue counters of the persistent batch launch, one slot per HIP
+  // stream that launches batches: launches on one stream are ordered, so a
+  // slot is never shared by two launches in flight (a slot taken over from
+  // another stream first waits for that stream's last launch on it).
+  uint32_t *d_counter = nullptr;  // kCounterSlots counters, kCounterStride bytes apart
+  struct CounterSlot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;  // recorded after the slot's last launch
+    uint64_t last_use = 0;
+    bool used = false;
+  } slots[kCounterSlots];
+  uint64_t use_clock = 0;
   uint32_t cu_count = 0;
   char name[256] = {0};
   codag_tune_params tune[4];
@@ -48,6 +68,7 @@ struct codag_ctx {
   const void *g_results = nullptr;
   uint32_t g_flags = 0, g_groups = 0;
   hipStream_t g_stream = nullptr;
+  int g_slot = -1;
   uint32_t poll_shift = 0;  // RLE v2 queue kernel: nonzero = waits without polls (tests)
 };
 
@@ -168,7 +189,9 @@ codag_status codag_ctx_create(int device, codag_ctx **out) {
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-  if (e == hipSuccess) e = hipMalloc(&c->d_counter, 256);
+  if (e == hipSuccess) e = hipMalloc(&c->d_counter, kCounterSlots * kCounterStride);
+  for (int i = 0; i < kCounterSlots && e == hipSuccess; i++)
+    e = hipEventCreateWithFlags(&c->slots[i].done, hipEventDisableTiming);
   if (e != hipSuccess) {
     codag_ctx_destroy(c);
     return fail(CODAG_ERR_HIP, std::string("ctx_create: ") + hipGetErrorString(e));
@@ -183,6 +206,11 @@ codag_status codag_ctx_destroy(codag_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (auto &sl : c->slots)
+    if (sl.done) {
+      if (sl.used) (void)hipEventSynchronize(sl.done);
+      (void)hipEventDestroy(sl.done);
+    }
   if (c->d_counter) (void)hipFree(c->d_counter);
   for (auto &e : c->evs)
     if (e) (void)hipEventDestroy(e);
@@ -264,6 +292,38 @@ codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_
   return CODAG_OK;
 }
 
+// The counter slot of stream s: the slot s already owns, else a free one,
+// else the least recently used, after waiting for its last launch (and
+// dropping the cached graph, which refers to the slot's counter).
+static codag_status counter_slot(codag_ctx *ctx, hipStream_t s, int *out) {
+  int pick_i = -1;
+  for (int i = 0; i < kCounterSlots; i++)
+    if (ctx->slots[i].used && ctx->slots[i].stream == s) pick_i = i;
+  if (pick_i < 0) {
+    for (int i = 0; i < kCounterSlots && pick_i < 0; i++)
+      if (!ctx->slots[i].used) pick_i = i;
+  }
+  if (pick_i < 0) {
+    pick_i = 0;
+    for (int i = 1; i < kCounterSlots; i++)
+      if (ctx->slots[i].last_use < ctx->slots[pick_i].last_use) pick_i = i;
+    HIP_TRY(hipEventSynchronize(ctx->slots[pick_i].done));
+    if (ctx->g_slot == pick_i) {
+      if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+      if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+      ctx->graph_exec = nullptr;
+      ctx->graph = nullptr;
+      ctx->g_slot = -1;
+    }
+  }
+  auto &sl = ctx->slots[pick_i];
+  sl.stream = s;
+  sl.used = true;
+  sl.last_use = ++ctx->use_clock;
+  *out = pick_i;
+  return CODAG_OK;
+}
+
 codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_descs, uint64_t n,
                                  codag_stream_result *d_results, uint32_t flags,
                                  codag_stream_t stream) {
@@ -276,11 +336,15 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
   uint32_t groups = ctx->cu_count * per_cu;
   if (n && groups > n) groups = (uint32_t)n;
   if (groups == 0) groups = 1;
+  int slot = -1;
+  if (codag_status st = counter_slot(ctx, s, &slot)) return st;
+  uint32_t *counter = ctx->d_counter + slot * (kCounterStride / sizeof(uint32_t));
   // Replay the captured graph when the same batch is launched again.
   if (ctx->graph_exec && ctx->g_descs == d_descs && ctx->g_n == n &&
       ctx->g_results == d_results && ctx->g_flags == flags && ctx->g_groups == groups &&
-      ctx->g_stream == s) {
+      ctx->g_stream == s && ctx->g_slot == slot) {
     HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
+    HIP_TRY(hipEventRecord(ctx->slots[slot].done, s));
     return CODAG_OK;
   }
   if (ctx->graph_exec) {
@@ -291,10 +355,11 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
     (void)hipGraphDestroy(ctx->graph);
     ctx->graph = nullptr;
   }
+  ctx->g_slot = -1;
   // Capture counter reset + persistent kernel into a graph on this stream.
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t le = launch_inflate_batch(d_descs, n, d_results, flags, ctx->d_counter, groups, s);
+    hipError_t le = launch_inflate_batch(d_descs, n, d_results, flags, counter, groups, s);
     e = hipStreamEndCapture(s, &ctx->graph);
     if (le != hipSuccess) e = le;
   }
@@ -306,14 +371,18 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
     ctx->g_flags = flags;
     ctx->g_groups = groups;
     ctx->g_stream = s;
+    ctx->g_slot = slot;
     HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
+    HIP_TRY(hipEventRecord(ctx->slots[slot].done, s));
     return CODAG_OK;
   }
   // Capture unavailable: launch directly.
   (void)hipGetLastError();
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   ctx->graph = nullptr;
   ctx->graph_exec = nullptr;
-  HIP_TRY(launch_inflate_batch(d_descs, n, d_results, flags, ctx->d_counter, groups, s));
+  HIP_TRY(launch_inflate_batch(d_descs, n, d_results, flags, counter, groups, s));
+  HIP_TRY(hipEventRecord(ctx->slots[slot].done, s));
   return CODAG_OK;
 }
 
@@ -378,6 +447,64 @@ codag_status codag_rlev1_decode_host(codag_ctx *ctx, const uint8_t *container,
   s = finish_host(ctx, d_st, n, kernel_seconds);
   if (total_seconds) *total_seconds = now_s() - t0;
   return s;
+}
+
+namespace {
+// Read-only map of a whole file; empty files map to nothing.
+struct MapFile {
+  void *p = nullptr;
+  uint64_t n = 0;
+  int fd = -1;
+  bool open(const char *path) {
+    fd = ::open(path, O_RDONLY);
+    if (fd < 0) return false;
+    struct stat sb;
+    if (fstat(fd, &sb) != 0) return false;
+    n = (uint64_t)sb.st_size;
+    if (!n) return true;
+    p = mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (p == MAP_FAILED) {
+      p = nullptr;
+      return false;
+    }
+    return true;
+  }
+  ~MapFile() {
+    if (p) munmap(p, n);
+    if (fd >= 0) ::close(fd);
+  }
+};
+}  // namespace
+
+codag_status codag_rlev1_decode_files(codag_ctx *ctx, const uint8_t *container,
+                                      uint64_t container_bytes, const char *col_len_path,
+                                      const char *blk_off_path, uint32_t w, uint64_t chunk_bytes,
+                                      uint8_t **out, uint64_t *out_bytes, uint64_t *meta_bytes,
+                                      double *kernel_seconds) {
+  if (!ctx || !container || !col_len_path || !blk_off_path || !out || !out_bytes)
+    return fail(CODAG_ERR_INVALID, "rlev1_decode_files: null argument");
+  *out = nullptr;
+  *out_bytes = 0;
+  if (container_bytes < 8) return fail(CODAG_ERR_FORMAT, "container shorter than its length word");
+  MapFile fc, fb;
+  if (!fc.open(col_len_path)) return fail(CODAG_ERR_IO, std::string("cannot read ") + col_len_path);
+  if (!fb.open(blk_off_path)) return fail(CODAG_ERR_IO, std::string("cannot read ") + blk_off_path);
+  if (meta_bytes) *meta_bytes = fc.n + fb.n;  // rlev1.h:501
+  const uint64_t orig = load_le(container, 8);  // rlev1.h:478
+  auto *buf = static_cast<uint8_t *>(std::malloc(orig ? orig : 1));
+  if (!buf) return fail(CODAG_ERR_NOMEM, "rlev1_decode_files: host output");
+  uint64_t got = 0;
+  codag_status s = codag_rlev1_decode_host(
+      ctx, container, container_bytes, static_cast<const uint64_t *>(fb.p), fb.n / 8,
+      static_cast<const uint64_t *>(fc.p), fc.n / 8, w, chunk_bytes, buf, orig, &got,
+      kernel_seconds, nullptr);
+  if (s) {
+    std::free(buf);
+    return s;
+  }
+  *out = buf;
+  *out_bytes = got;
+  return CODAG_OK;
 }
 
 codag_status codag_rlev2_decode_host(codag_ctx *ctx, const uint8_t *comp, uint64_t comp_bytes,
@@ -637,6 +764,45 @@ codag_status codag_memset_d(codag_ctx *ctx, void *d_dst, int value, uint64_t byt
   if (!bytes) return CODAG_OK;
   DeviceGuard g(ctx->device);
   HIP_TRY(hipMemsetAsync(d_dst, value, bytes, ctx->stream));
+  return CODAG_OK;
+}
+
+codag_status codag_stream_create(codag_ctx *ctx, codag_stream_t *out) {
+  if (!ctx || !out) return fail(CODAG_ERR_INVALID, "stream_create: null argument");
+  DeviceGuard g(ctx->device);
+  hipStream_t s = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *out = s;
+  return CODAG_OK;
+}
+
+codag_status codag_stream_destroy(codag_ctx *ctx, codag_stream_t stream) {
+  if (!ctx || !stream) return fail(CODAG_ERR_INVALID, "stream_destroy: null argument");
+  DeviceGuard g(ctx->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamSynchronize(s));
+  // a counter slot or cached graph bound to this stream is released with it
+  for (auto &sl : ctx->slots)
+    if (sl.used && sl.stream == s) {
+      sl.used = false;
+      sl.stream = nullptr;
+    }
+  if (ctx->g_stream == s) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->g_stream = nullptr;
+    ctx->g_slot = -1;
+  }
+  HIP_TRY(hipStreamDestroy(s));
+  return CODAG_OK;
+}
+
+codag_status codag_stream_sync(codag_ctx *ctx, codag_stream_t stream) {
+  if (!ctx) return fail(CODAG_ERR_INVALID, "stream_sync: null ctx");
+  DeviceGuard g(ctx->device);
+  HIP_TRY(hipStreamSynchronize(pick(ctx, stream)));
   return CODAG_OK;
 }
 

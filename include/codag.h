@@ -117,7 +117,11 @@ codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_
 /* Uneven batch of independent zlib streams (no reference counterpart: the
  * reference has fixed-size chunks only; shape follows cudf_deflate.cuh:236-250).
  * One persistent launch pulls streams from a device work queue in descriptor
- * order (listing the largest streams first gives the best load balance). */
+ * order (listing the largest streams first gives the best load balance).
+ * The work queue's counter belongs to the HIP stream the batch is launched
+ * on, so batches of one context on different streams may overlap; a context
+ * serves up to 16 streams at once (a 17th waits for the least recently used
+ * stream's last batch). */
 typedef struct codag_stream_desc {
   const uint8_t *src; /* device */
   uint64_t src_bytes;
@@ -152,6 +156,17 @@ codag_status codag_rlev1_decode_host(codag_ctx *ctx, const uint8_t *container,
                                      uint64_t *out_bytes, double *kernel_seconds,
                                      double *total_seconds);
 
+/* RLE v1, side files read from disk as rle_v1::decompress_gpu_orig does
+ * (rlev1.h:436-467, normally ./input_data/rle_v1_col_len.bin and
+ * ./input_data/rle_v1_blk_offset.bin). *out receives the original length of
+ * decoded bytes in a malloc'd buffer (free() or codag_host_free);
+ * meta_bytes (nullable) the two files' sizes, as rlev1.h:501 counts them. */
+codag_status codag_rlev1_decode_files(codag_ctx *ctx, const uint8_t *container,
+                                      uint64_t container_bytes, const char *col_len_path,
+                                      const char *blk_off_path, uint32_t elem_bytes,
+                                      uint64_t chunk_bytes, uint8_t **out, uint64_t *out_bytes,
+                                      uint64_t *meta_bytes, double *kernel_seconds);
+
 /* RLE v2: comp = concatenated chunk streams; blk_off has n_chunks+1 entries
  * with blk_off[n_chunks] = original byte length (encoder_trans.h:545);
  * col_len has >= n_chunks entries. */
@@ -170,8 +185,8 @@ codag_status codag_inflate_chunks_host(codag_ctx *ctx, const uint8_t *comp, uint
                                        double *kernel_seconds, double *total_seconds);
 
 /* --------------------------------------------------- producers (CPU, C++) */
-/* Buffers returned by producers are allocated by the library; release them
- * with codag_host_free. */
+/* Buffers returned by producers are allocated by the library with malloc;
+ * release them with codag_host_free (or free). */
 void codag_host_free(void *p);
 
 /* RLE v1 container writer (role of rle_v1::compress_gpu_orig, rlev1.h:103-192):
@@ -181,7 +196,11 @@ void codag_host_free(void *p);
  * first n_chunks+1 meaningful, the rest repeat the total); col_len
  * (n_chunks*32*32 u64, the first n_chunks meaningful, the rest zero) -- the
  * sizes the reference writes to ./input_data/ (rlev1.h:178-186).
- * Byte-identity with the reference encoder's choices is not claimed. */
+ * The chunk streams are the reference encoder's bytes (compression_warp_orig,
+ * encoder.h:1352-1695): same runs, literal groups and header bytes; the
+ * padding bytes between streams and the side-file entries past the first
+ * n_chunks(+1), which the reference leaves uninitialised, are zero / repeat
+ * the total here. */
 codag_status codag_rlev1_encode(const uint8_t *in, uint64_t in_bytes, uint32_t elem_bytes,
                                 uint64_t chunk_bytes, uint8_t **container,
                                 uint64_t *container_bytes, uint64_t **blk_off,
@@ -190,7 +209,13 @@ codag_status codag_rlev1_encode(const uint8_t *in, uint64_t in_bytes, uint32_t e
 
 /* RLE v2 container writer (role of rlev2::compress_gpu_orig, encoder_trans.h:
  * 482-564): n_chunks rounded up to a multiple of 32, streams aligned to 128 B,
- * blk_off has n_chunks+1 entries with the last = in_bytes, col_len n_chunks. */
+ * blk_off has n_chunks+1 entries with the last = in_bytes, col_len n_chunks.
+ * The chunk streams are the reference encoder's bytes (block_encode_orig,
+ * encoder_trans.h:184-357, over encoder.h:83-591), with one exception: a run
+ * of 128 zeros, for which the reference writes a DIRECT block its decoder
+ * cannot read, is written as a fixed DELTA block. As in the reference,
+ * 64-bit columns whose PATCHED_BASE patches exceed 32 bits do not round-trip
+ * (the decoder keeps 32 bits of a patch, decoder_new.h:1105). */
 codag_status codag_rlev2_encode(const uint8_t *in, uint64_t in_bytes, uint32_t elem_bytes,
                                 uint64_t chunk_bytes, uint8_t **comp, uint64_t *comp_bytes,
                                 uint64_t **blk_off, uint64_t **col_len, uint64_t *n_chunks);
@@ -274,11 +299,16 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
                             uint8_t *d_out, int reps, codag_tune_params *best, float *best_ms);
 
 /* ---------------------------------------------------------------- serving */
-/* Continuous-batching decode server (no reference counterpart). A host
- * scheduler thread admits requests against an HBM byte budget and a slot
- * limit, copies inputs to device-resident buffers, launches each admitted
- * request's decode on its own HIP stream so work overlaps on the device, and
- * reports per-request latency. */
+/* Continuous-batching decode server (no reference counterpart). max_inflight
+ * worker slots, each with its own context and HIP stream, admit queued
+ * requests against an HBM byte budget: a free slot takes the first request in
+ * arrival order whose device footprint fits the budget left. A request that
+ * does not fit can be passed by later, smaller requests at most 8 times;
+ * after that nothing passes it until it is admitted. Host-I/O requests are
+ * copied into stream-ordered buffers of the server's own device memory pool
+ * (no device-wide synchronisation once the pool holds the sizes), each
+ * admitted request's decode runs on its slot's stream so work overlaps on
+ * the device, and every completion reports its latency. */
 typedef struct codag_server codag_server;
 
 typedef struct codag_server_config {
@@ -319,9 +349,9 @@ typedef struct codag_completion {
 } codag_completion;
 
 codag_status codag_server_create(const codag_server_config *cfg, codag_server **out);
-/* Non-blocking. CODAG_ERR_BUSY never happens here: requests wait in a FIFO
- * until admitted; a request larger than the whole budget is rejected with
- * CODAG_ERR_INVALID. */
+/* Non-blocking. CODAG_ERR_BUSY never happens here: requests wait in the
+ * queue until admitted; a request larger than the whole budget is rejected
+ * with CODAG_ERR_INVALID. */
 codag_status codag_server_submit(codag_server *s, const codag_request *req, uint64_t *request_id);
 /* Wait up to timeout_ms (-1 = forever) for at least one completion; returns up
  * to max completions. */
@@ -345,6 +375,12 @@ codag_status codag_memset_d(codag_ctx *ctx, void *d_dst, int value, uint64_t byt
  * reference point when HBM traffic counters are scaled. */
 codag_status codag_copy_d2d(codag_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);
 codag_status codag_sync(codag_ctx *ctx);
+/* Extra HIP streams (default priority, non-blocking) on the context's device,
+ * for callers without their own HIP bindings that want decodes to overlap.
+ * Destroying a stream waits for its work. */
+codag_status codag_stream_create(codag_ctx *ctx, codag_stream_t *out);
+codag_status codag_stream_destroy(codag_ctx *ctx, codag_stream_t stream);
+codag_status codag_stream_sync(codag_ctx *ctx, codag_stream_t stream);
 /* HIP events recorded on the context stream around a region. */
 codag_status codag_timer_start(codag_ctx *ctx);
 codag_status codag_timer_stop(codag_ctx *ctx, float *elapsed_ms);

@@ -1,6 +1,7 @@
 """GPU parity of the Deflate decoder (codag_inflate_chunks / codag_inflate_batch)
 against zlib's inflate (the oracle): byte-identical output, same byte counts,
 same status codes, and the Adler-32 the kernel computes equals zlib's."""
+import ctypes
 import zlib
 
 import numpy as np
@@ -235,36 +236,53 @@ def test_host_wrapper_reference_shape(ctx):
     assert out.tobytes() == raw.tobytes() and secs > 0
 
 
+class Batch:
+    """Device-resident batch: streams, descriptors, outputs and results are
+    uploaded/allocated once, so repeated launches use the same addresses."""
+
+    def __init__(self, ctx, streams, caps):
+        self.ctx, self.n = ctx, len(streams)
+        src = np.frombuffer(b"".join(streams) or b"\0", np.uint8)
+        soff = np.cumsum([0] + [len(s) for s in streams]).astype(np.uint64)
+        self.doff = np.cumsum([0] + list(caps)).astype(np.uint64)
+        self.d_src = ctx.upload(src)
+        self.d_dst = ctx.alloc(int(self.doff[-1]) or 1)
+        self.descs = (ca.StreamDesc * self.n)()
+        for i in range(self.n):
+            self.descs[i].src = self.d_src.offset(int(soff[i]))
+            self.descs[i].src_bytes = len(streams[i])
+            self.descs[i].dst = self.d_dst.offset(int(self.doff[i]))
+            self.descs[i].dst_cap = int(caps[i])
+        self.d_desc = ctx.upload(np.frombuffer(bytes(self.descs), np.uint8))
+        self.d_res = ctx.alloc(16 * max(self.n, 1))
+
+    def launch(self, flags=0, stream=None, clear=True):
+        if clear:
+            self.d_dst.fill(0)
+            self.d_res.fill(0x7F)
+        self.ctx.inflate_batch(self.d_desc, self.n, self.d_res, flags, stream)
+
+    def results(self):
+        res = self.d_res.download(16 * self.n).view(np.uint8).reshape(self.n, 16)
+        written = res[:, :8].copy().view(np.uint64).reshape(-1)
+        status = res[:, 8:12].copy().view(np.int32).reshape(-1)
+        adler = res[:, 12:16].copy().view(np.uint32).reshape(-1)
+        return status, written, adler, self.d_dst.download(), self.doff
+
+
 def _batch(ctx, streams, caps, flags=0):
-    src = np.frombuffer(b"".join(streams) or b"\0", np.uint8)
-    soff = np.cumsum([0] + [len(s) for s in streams]).astype(np.uint64)
-    doff = np.cumsum([0] + list(caps)).astype(np.uint64)
-    d_src = ctx.upload(src)
-    d_dst = ctx.alloc(int(doff[-1]) or 1)
-    d_dst.fill(0)
-    descs = (ca.StreamDesc * len(streams))()
-    for i in range(len(streams)):
-        descs[i].src = d_src.offset(int(soff[i]))
-        descs[i].src_bytes = len(streams[i])
-        descs[i].dst = d_dst.offset(int(doff[i]))
-        descs[i].dst_cap = int(caps[i])
-    d_desc = ctx.upload(np.frombuffer(bytes(descs), np.uint8))
-    d_res = ctx.alloc(16 * len(streams))
-    d_res.fill(0x7F)
-    ctx.inflate_batch(d_desc, len(streams), d_res, flags)
+    b = Batch(ctx, streams, caps)
+    b.launch(flags)
     ctx.sync()
-    res = d_res.download().view(np.uint8).reshape(len(streams), 16)
-    written = res[:, :8].copy().view(np.uint64).reshape(-1)
-    status = res[:, 8:12].copy().view(np.int32).reshape(-1)
-    adler = res[:, 12:16].copy().view(np.uint32).reshape(-1)
-    return status, written, adler, d_dst.download(), doff
+    return b.results()
 
 
-def test_batch_uneven_streams(ctx):
-    say("inflate batch uneven")
-    rng = np.random.default_rng(10)
-    sizes = np.exp(rng.uniform(np.log(4096), np.log(600000), 60)).astype(int)
-    payload = ca.synth_deflate_payload(int(sizes.sum()), CH, seed=10)
+def _uneven(seed, count, hi, big=()):
+    """`count` zlib streams log-uniform in 4 KiB..hi plus streams of the sizes
+    in `big`, with the fixed/dynamic/stored block mix 20/75/5."""
+    rng = np.random.default_rng(seed)
+    sizes = list(np.exp(rng.uniform(np.log(4096), np.log(hi), count)).astype(int)) + list(big)
+    payload = ca.synth_deflate_payload(int(sum(sizes)), CH, seed=seed)
     streams, raws, pos = [], [], 0
     for i, n in enumerate(sizes):
         r = payload[pos: pos + n].tobytes()
@@ -272,13 +290,74 @@ def test_batch_uneven_streams(ctx):
         raws.append(r)
         s, counts = ca.deflate_compress_mixed(np.frombuffer(r, np.uint8), 20, 5, 32768, seed=i)
         streams.append(s.tobytes())
-    caps = [len(r) for r in raws]
-    for rep in range(3):  # the second and third calls replay the captured graph
-        status, written, adler, dst, doff = _batch(ctx, streams, caps)
-        for i, r in enumerate(raws):
-            assert status[i] == 0 and int(written[i]) == len(r)
-            assert dst[int(doff[i]): int(doff[i]) + len(r)].tobytes() == r
-            assert int(adler[i]) == zlib.adler32(r)
+    return streams, raws
+
+
+def _check_batch(res, raws):
+    status, written, adler, dst, doff = res
+    for i, r in enumerate(raws):
+        assert status[i] == 0 and int(written[i]) == len(r), (i, status[i], written[i], len(r))
+        assert dst[int(doff[i]): int(doff[i]) + len(r)].tobytes() == r, i
+        assert int(adler[i]) == zlib.adler32(r), i
+
+
+def test_batch_uneven_streams(ctx):
+    say("inflate batch uneven, streams up to 4 MiB")
+    # two 4 MiB streams (the largest size of the uneven-batch workload) and
+    # 60 smaller ones, largest first as the work queue prefers
+    streams, raws = _uneven(10, 60, 600000, big=(4 << 20, (4 << 20) - 12345))
+    order = sorted(range(len(raws)), key=lambda i: -len(raws[i]))
+    streams = [streams[i] for i in order]
+    raws = [raws[i] for i in order]
+    _check_batch(_batch(ctx, streams, [len(r) for r in raws]), raws)
+
+
+def test_batch_graph_replay(ctx):
+    say("inflate batch graph replay on fixed buffers")
+    streams, raws = _uneven(12, 40, 300000)
+    b = Batch(ctx, streams, [len(r) for r in raws])
+    for rep in range(3):  # the same (descs, n, results) replays the captured graph
+        b.launch()
+        ctx.sync()
+        _check_batch(b.results(), raws)
+    # new descriptor contents at the same address: the replay reads them
+    rev = list(range(b.n))[::-1]
+    moved = (ca.StreamDesc * b.n)()
+    for k, i in enumerate(rev):
+        moved[k] = b.descs[i]
+    ca.check(ca.lib().codag_copy_h2d(ctx.h, b.d_desc.ptr, ctypes.cast(moved, ctypes.c_void_p),
+                                     ctypes.sizeof(moved)), "copy_h2d")
+    b.launch()
+    ctx.sync()
+    status, written, adler, dst, doff = b.results()
+    for k, i in enumerate(rev):  # result k now describes stream i
+        assert status[k] == 0 and int(written[k]) == len(raws[i])
+        assert int(adler[k]) == zlib.adler32(raws[i])
+        assert dst[int(doff[i]): int(doff[i]) + len(raws[i])].tobytes() == raws[i]
+
+
+def test_batch_two_streams_one_context(ctx):
+    say("inflate batch on two HIP streams of one context")
+    sa, ra = _uneven(13, 50, 400000)
+    sb, rb = _uneven(14, 70, 200000)
+    a = Batch(ctx, sa, [len(r) for r in ra])
+    b = Batch(ctx, sb, [len(r) for r in rb])
+    s1, s2 = ctx.stream_create(), ctx.stream_create()
+    try:
+        for rep in range(2):
+            for x in (a, b):
+                x.d_dst.fill(0)
+                x.d_res.fill(0x7F)
+            ctx.sync()
+            a.launch(stream=s1, clear=False)
+            b.launch(stream=s2, clear=False)
+            ctx.stream_sync(s1)
+            ctx.stream_sync(s2)
+            _check_batch(a.results(), ra)
+            _check_batch(b.results(), rb)
+    finally:
+        ctx.stream_destroy(s1)
+        ctx.stream_destroy(s2)
 
 
 def test_batch_errors_and_overflow(ctx):
