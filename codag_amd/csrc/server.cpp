@@ -1,13 +1,16 @@
 // server.cpp -- continuous-batching decode server over the C ABI.
 //
-// Requests (RLE v1 / RLE v2 / Deflate containers in host memory) enter a FIFO.
-// Each of max_inflight worker slots owns a HIP stream (default priority) and
-// grow-only device buffers. A slot admits the FIFO head as soon as the head's
-// device footprint fits in the remaining HBM budget, copies it in, launches
-// the decode on its own stream and copies the result out, so requests that
+// Requests (RLE v1 / RLE v2 / Deflate containers in host or device memory)
+// enter a queue. Each of max_inflight worker slots owns a codag context (its
+// own HIP stream, default priority) and stream-ordered device buffers from
+// the server's memory pool. A free slot admits the first queued request
+// whose device footprint fits the remaining HBM budget, so requests that
 // arrive while others run start as soon as a slot and budget free up and
-// their kernels overlap on the device. Latency is measured from submit to
-// the result being in host memory.
+// their kernels overlap on the device. A request that does not fit may be
+// passed by later, smaller ones at most kMaxPass times; after that nothing
+// passes it until it is admitted (the aging bound). Latency is measured from
+// submit to the result being in host memory (device-resident requests: to
+// the decode and its status check being complete).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -25,6 +28,10 @@ using namespace codag;
 
 namespace {
 
+// A request blocked on the budget lets at most this many later requests be
+// admitted ahead of it.
+constexpr uint32_t kMaxPass = 8;
+
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -33,23 +40,33 @@ struct Pending {
   uint64_t id;
   codag_request req;
   double t_submit;
-  uint64_t bytes;  // device footprint charged against the budget
+  uint64_t bytes;       // device footprint charged against the budget
+  uint32_t passed = 0;  // later requests admitted ahead of this one
 };
 
+// A stream-ordered device buffer of one worker slot. It is taken from the
+// server's memory pool on the slot's stream and returned to it when a larger
+// one is needed, so a slot never calls hipMalloc/hipFree (which synchronize
+// the device and would stall the other slots) once the pool holds its sizes.
 struct Grow {
   void *p = nullptr;
   uint64_t cap = 0;
+  hipMemPool_t pool = nullptr;
+  hipStream_t st = nullptr;
   hipError_t ensure(uint64_t n) {
     if (n <= cap && p) return hipSuccess;
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFreeAsync(p, st);
     p = nullptr;
     cap = 0;
-    hipError_t e = hipMalloc(&p, n ? n : 1);
-    if (e == hipSuccess) cap = n;
+    const uint64_t want = n ? n : 1;
+    hipError_t e = pool ? hipMallocFromPoolAsync(&p, want, pool, st) : hipMallocAsync(&p, want, st);
+    if (e == hipSuccess) cap = want;
     return e;
   }
-  ~Grow() {
-    if (p) (void)hipFree(p);
+  void release() {
+    if (p) (void)hipFreeAsync(p, st);
+    p = nullptr;
+    cap = 0;
   }
 };
 
@@ -62,7 +79,8 @@ uint64_t footprint(const codag_request &r) {
 
 struct codag_server {
   codag_server_config cfg{};
-  codag_ctx *ctx = nullptr;
+  std::vector<codag_ctx *> ctxs;  // one per worker slot
+  hipMemPool_t pool = nullptr;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Pending> queue;
@@ -72,9 +90,20 @@ struct codag_server {
   bool stop = false;
   uint64_t next_id = 1;
   std::vector<std::thread> workers;
+
+  // Index of the queued request to admit next, or -1: the first one in
+  // queue order that fits the budget, not passing a request that has been
+  // passed kMaxPass times. Called with mu held.
+  long admissible() const {
+    for (size_t i = 0; i < queue.size(); i++) {
+      if (admitted + queue[i].bytes <= cfg.hbm_budget_bytes) return (long)i;
+      if (queue[i].passed >= kMaxPass) return -1;
+    }
+    return -1;
+  }
 };
 
-static codag_status run_one(codag_server *s, hipStream_t st, Grow *b, const Pending &p,
+static codag_status run_one(codag_ctx *ctx, hipStream_t st, Grow *b, const Pending &p,
                             uint32_t *failed) {
   const codag_request &r = p.req;
   const uint64_t n = r.n_chunks;
@@ -86,13 +115,13 @@ static codag_status run_one(codag_server *s, hipStream_t st, Grow *b, const Pend
     int32_t *stat = static_cast<int32_t *>(b[4].p);
     codag_status cs;
     if (r.codec == CODAG_RLEV1)
-      cs = codag_rlev1_decode(s->ctx, r.comp, r.comp_bytes, r.blk_off, r.col_len, n, r.elem_bytes,
+      cs = codag_rlev1_decode(ctx, r.comp, r.comp_bytes, r.blk_off, r.col_len, n, r.elem_bytes,
                               r.chunk_bytes, r.out, stat, st);
     else if (r.codec == CODAG_RLEV2)
-      cs = codag_rlev2_decode(s->ctx, r.comp, r.comp_bytes, r.blk_off, r.col_len, n, r.elem_bytes,
+      cs = codag_rlev2_decode(ctx, r.comp, r.comp_bytes, r.blk_off, r.col_len, n, r.elem_bytes,
                               r.chunk_bytes, r.out, stat, st);
     else
-      cs = codag_inflate_chunks(s->ctx, r.comp, r.comp_bytes, r.blk_off, n, r.chunk_bytes, r.out,
+      cs = codag_inflate_chunks(ctx, r.comp, r.comp_bytes, r.blk_off, n, r.chunk_bytes, r.out,
                                 nullptr, stat, 0, st);
     if (cs) return cs;
     std::vector<int32_t> hs(n);
@@ -123,13 +152,13 @@ static codag_status run_one(codag_server *s, hipStream_t st, Grow *b, const Pend
   uint8_t *out = static_cast<uint8_t *>(b[3].p);
   int32_t *stat = static_cast<int32_t *>(b[4].p);
   if (r.codec == CODAG_RLEV1)
-    cs = codag_rlev1_decode(s->ctx, comp, r.comp_bytes, off, len, n, r.elem_bytes, r.chunk_bytes,
+    cs = codag_rlev1_decode(ctx, comp, r.comp_bytes, off, len, n, r.elem_bytes, r.chunk_bytes,
                             out, stat, st);
   else if (r.codec == CODAG_RLEV2)
-    cs = codag_rlev2_decode(s->ctx, comp, r.comp_bytes, off, len, n, r.elem_bytes, r.chunk_bytes,
+    cs = codag_rlev2_decode(ctx, comp, r.comp_bytes, off, len, n, r.elem_bytes, r.chunk_bytes,
                             out, stat, st);
   else
-    cs = codag_inflate_chunks(s->ctx, comp, r.comp_bytes, off, n, r.chunk_bytes, out, nullptr,
+    cs = codag_inflate_chunks(ctx, comp, r.comp_bytes, off, n, r.chunk_bytes, out, nullptr,
                               stat, 0, st);
   if (cs) return cs;
   std::vector<int32_t> hs(n);
@@ -143,26 +172,29 @@ static codag_status run_one(codag_server *s, hipStream_t st, Grow *b, const Pend
   return bad ? CODAG_ERR_DECODE : CODAG_OK;
 }
 
-static void worker(codag_server *s) {
+static void worker(codag_server *s, uint32_t slot) {
+  codag_ctx *ctx = s->ctxs[slot];
   (void)hipSetDevice(s->cfg.device);
-  hipStream_t st = nullptr;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return;
+  hipStream_t st = static_cast<hipStream_t>(codag_ctx_stream(ctx));
   Grow bufs[5];
+  for (auto &g : bufs) {
+    g.pool = s->pool;
+    g.st = st;
+  }
   for (;;) {
     std::unique_lock<std::mutex> lk(s->mu);
-    s->cv_work.wait(lk, [&] {
-      return s->stop ||
-             (!s->queue.empty() && s->admitted + s->queue.front().bytes <= s->cfg.hbm_budget_bytes);
-    });
+    long k = -1;
+    s->cv_work.wait(lk, [&] { return s->stop || (k = s->admissible()) >= 0; });
     if (s->stop) break;
-    Pending p = s->queue.front();
-    s->queue.pop_front();
+    for (long i = 0; i < k; i++) s->queue[i].passed++;
+    Pending p = s->queue[k];
+    s->queue.erase(s->queue.begin() + k);
     s->admitted += p.bytes;
     s->inflight++;
     lk.unlock();
     const double t_adm = now_s();
     uint32_t failed = 0;
-    codag_status cs = run_one(s, st, bufs, p, &failed);
+    codag_status cs = run_one(ctx, st, bufs, p, &failed);
     const double t_end = now_s();
     lk.lock();
     s->admitted -= p.bytes;
@@ -178,8 +210,8 @@ static void worker(codag_server *s) {
     s->cv_done.notify_all();
     s->cv_work.notify_all();
   }
+  for (auto &g : bufs) g.release();
   (void)hipStreamSynchronize(st);
-  (void)hipStreamDestroy(st);
 }
 
 extern "C" {
@@ -192,12 +224,36 @@ codag_status codag_server_create(const codag_server_config *cfg, codag_server **
   auto *s = new (std::nothrow) codag_server();
   if (!s) return fail(CODAG_ERR_NOMEM, "server_create");
   s->cfg = *cfg;
-  codag_status cs = codag_ctx_create(cfg->device, &s->ctx);
-  if (cs) {
-    delete s;
-    return cs;
+  for (uint32_t i = 0; i < cfg->max_inflight; i++) {
+    codag_ctx *c = nullptr;
+    codag_status cs = codag_ctx_create(cfg->device, &c);
+    if (cs) {
+      for (auto *x : s->ctxs) codag_ctx_destroy(x);
+      delete s;
+      return cs;
+    }
+    s->ctxs.push_back(c);
   }
-  for (uint32_t i = 0; i < cfg->max_inflight; i++) s->workers.emplace_back(worker, s);
+  {
+    // The server's own stream-ordered pool: freed blocks stay in it (release
+    // threshold: never), so steady-state admissions allocate nothing.
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(cfg->device);
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = cfg->device;
+    if (hipMemPoolCreate(&s->pool, &props) == hipSuccess) {
+      uint64_t thr = ~0ull;
+      (void)hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &thr);
+    } else {
+      (void)hipGetLastError();
+      s->pool = nullptr;  // fall back to the device's default pool
+    }
+    (void)hipSetDevice(prev);
+  }
+  for (uint32_t i = 0; i < cfg->max_inflight; i++) s->workers.emplace_back(worker, s, i);
   *out = s;
   return CODAG_OK;
 }
@@ -267,7 +323,8 @@ codag_status codag_server_destroy(codag_server *s) {
   }
   s->cv_work.notify_all();
   for (auto &t : s->workers) t.join();
-  codag_ctx_destroy(s->ctx);
+  for (auto *c : s->ctxs) codag_ctx_destroy(c);
+  if (s->pool) (void)hipMemPoolDestroy(s->pool);
   delete s;
   return CODAG_OK;
 }

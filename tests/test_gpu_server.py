@@ -142,3 +142,49 @@ def test_admission_budget(gpu_required):
             srv2.close()
     finally:
         srv.close()
+
+
+def _deflate_request(n_chunks, seed):
+    raw = ca.synth_deflate_payload(n_chunks * CH, CH, seed=seed)
+    comp, blk, _, n = ca.deflate_compress_chunks(raw, CH, level=6, threads=8)
+    return dict(codec=ca.DEFLATE, w=1, n=n, comp=np.ascontiguousarray(comp), off=blk[1:].copy(),
+                ln=np.zeros(1, np.uint64), out=np.zeros(n * CH, np.uint8), raw=raw)
+
+
+def _footprint(r):
+    meta = (r["n"] + 1) * 8 if r["codec"] == ca.DEFLATE else r["n"] * 16
+    return len(r["comp"]) + meta + r["n"] * CH + r["n"] * 4
+
+
+def test_blocked_head_is_passed_within_the_aging_bound(gpu_required):
+    say("server: 40 MiB requests, a budget-blocked head, small requests passing it")
+    big1, big2 = _deflate_request(320, 51), _deflate_request(320, 52)  # 40 MiB each
+    smalls = [_deflate_request(1, 60 + i) for i in range(12)]
+    fb = max(_footprint(big1), _footprint(big2))
+    budget = fb + 8 * (1 << 20)  # one big request plus every small one, never both bigs
+    assert _footprint(big1) + _footprint(big2) > budget
+    assert fb + sum(_footprint(r) for r in smalls) <= budget
+    srv = Server(budget, 6)
+    try:
+        import time
+        t_sub = {}
+        reqs = [big1, big2] + smalls
+        structs = [to_struct(r, i) for i, r in enumerate(reqs)]
+        for i, q in enumerate(structs):
+            t_sub[i] = time.perf_counter()
+            assert srv.submit(q)[0] == ca.OK
+        done = {c.user_tag: c for c in srv.wait_all(len(reqs))}
+        assert all(c.status == ca.OK and c.failed_chunks == 0 for c in done.values())
+        for r in reqs:
+            assert r["out"][: len(r["raw"])].tobytes() == r["raw"].tobytes()
+        # admission time of request i, on the submitter's clock (the server's
+        # clock differs by a constant, which cancels in the comparisons)
+        adm = {i: t_sub[i] + done[i].queue_seconds for i in done}
+        # big2 cannot start before big1 has finished (the budget holds one)
+        assert adm[1] >= t_sub[0] + done[0].latency_seconds - 2e-3
+        # the first kMaxPass (8) small requests may pass the blocked big2;
+        # the 9th to 12th are never admitted before it
+        for i in range(2 + 8, 2 + 12):
+            assert adm[i] >= adm[1] - 2e-3, (i, adm[i] - adm[1])
+    finally:
+        srv.close()
