@@ -6,14 +6,15 @@
 //   RLE v2 stream/container : include/rlev2/decoder_new.h:909-1187,
 //                             encoder_trans.h:482-564 (128-byte alignment)
 //   Deflate container       : deflate_compressor.py:44-126
-// The encoding *choices* are this project's own (greedy); decode parity does
-// not depend on them and byte identity with the reference encoders is not
-// claimed.
+// codag_rlev1_encode restates the reference's RLE v1 encoder choices, so its
+// streams are the reference's byte for byte; the synthetic column generators
+// emit their own controlled segment mixes.
 #include "../internal.hpp"
 
 #include <zlib.h>
 
 #include <algorithm>
+#include <memory>
 #include <atomic>
 #include <cmath>
 #include <string>
@@ -107,6 +108,218 @@ static std::string rlev1_encode_chunk(const uint8_t *in, uint64_t n, uint32_t w)
   }
   flush();
   return out;
+}
+
+// The reference's RLE v1 chunk encoder, restated for byte-identical streams:
+// the writer state machine of compression_warp_orig (include/rlev1/
+// encoder.h:1352-1695), one chunk of exactly n values (the reference encodes
+// the zero-padded column a full chunk at a time, encoder.h:60-74). Its
+// choices, all kept here:
+//  * a run needs three values with one signed 8-bit step; the step and the
+//    run test are computed in the column type T before widening, with C++'s
+//    promotions (encoder.h:1416, 1459, 1491): uint8/uint16 steps never wrap,
+//    uint32 steps are unsigned (a decrease is never a run), uint64 steps
+//    wrap to a signed 64-bit difference;
+//  * runs hold 3..129 values: the 130th value closes the run (header 126)
+//    and starts the next sequence (encoder.h:1509-1523);
+//  * literal groups hold at most 127 values (encoder.h:1447, 1573); the
+//    group header is a placeholder byte (0x01 or 0x03, as the reference
+//    writes them) overwritten with -count when the group closes;
+//  * values and run bases are unsigned LEB128 of the T value
+//    (write_varint_op_orig, encoder.h:839-853).
+template <typename T> static std::string rlev1_ref_encode_chunk(const T *v, uint64_t n) {
+  std::string out;
+  auto byte = [&](uint8_t b) { out.push_back(static_cast<char>(b)); };
+  auto varint = [&](T val) {
+    do {
+      uint8_t b = (uint8_t)(val & 0x7F);
+      if ((val & ~(T)0x7F) != 0) b |= 0x80;
+      byte(b);
+      val = (T)(val >> 7);
+    } while (val != 0);
+  };
+  auto in_range = [](int64_t d) { return !(d > 127 || d < -128); };
+  T buf[2] = {0, 0};
+  uint32_t head = 0, count = 0;
+  T first = 0, prev = 0;
+  uint32_t dcount = 0;
+  int8_t cur = 0;
+  bool dflag = false;
+  uint64_t lit_idx = 0;
+  uint32_t lit = 0;
+  auto push = [&](T x) {  // buffer x as the newest value
+    buf[head] = x;
+    head ^= 1u;
+  };
+  auto close_group = [&]() { out[lit_idx] = static_cast<char>(static_cast<int8_t>(-(int)lit)); };
+  if (n == 0) return out;
+  // first value
+  T x = v[0];
+  dcount = 1;
+  prev = x;
+  push(x);
+  count = 1;
+  uint64_t i = 1;
+  if (n > 1) {  // second value (the reference always reads two)
+    x = v[1];
+    i = 2;
+    const int64_t d = (int64_t)(x - prev);
+    if (!in_range(d)) {
+      dflag = false;
+      lit_idx = out.size();
+      varint((T)1);
+      lit = 1;
+      varint(buf[0]);
+      count--;
+    } else {
+      dflag = true;
+      cur = (int8_t)d;
+      dcount++;
+    }
+    prev = x;
+    push(x);
+    count++;
+  }
+  for (; i < n; i++) {
+    if (lit == 127) {
+      close_group();
+      lit = 0;
+    }
+    x = v[i];
+    if (dcount == 1) {
+      const int64_t d = (int64_t)(x - prev);
+      if (!in_range(d)) {
+        dflag = false;
+        if (lit == 0) {
+          lit_idx = out.size();
+          varint((T)1);
+        }
+        varint(buf[head ^ 1u]);
+        lit++;
+        count--;
+      } else {
+        dflag = true;
+        cur = (int8_t)d;
+        dcount++;
+      }
+      prev = x;
+      push(x);
+      count++;
+      continue;
+    }
+    if (prev + cur == x && dflag) {  // in T with C++ promotions (encoder.h:1491)
+      dcount++;
+      if (dcount == 3) {
+        first = buf[head];
+        if (lit != 0) {
+          close_group();
+          lit = 0;
+        }
+      } else if (dcount == 130) {
+        byte((uint8_t)(dcount - 4));
+        byte((uint8_t)cur);
+        varint(first);
+        dcount = 1;
+        count = 0;
+        lit = 0;
+      }
+      push(x);
+      count = std::min(count + 1, 2u);
+      prev = x;
+    } else if (dcount >= 3) {
+      byte((uint8_t)(dcount - 3));
+      byte((uint8_t)cur);
+      varint(first);
+      dcount = 1;
+      lit = 0;
+      push(x);
+      count = 1;
+      prev = x;
+    } else {
+      if (lit == 0) {
+        lit_idx = out.size();
+        byte(3);
+      }
+      lit++;
+      varint(buf[head]);
+      if (lit == 127) {
+        close_group();
+        lit = 0;
+      }
+      const int64_t d = (int64_t)(x - prev);
+      if (!in_range(d)) {
+        if (lit == 0) {
+          lit_idx = out.size();
+          byte(3);
+        }
+        dflag = false;
+        count = 0;
+        varint(buf[head ^ 1u]);
+        lit++;
+        dcount = 1;
+      } else {
+        count = 1;
+        dflag = true;
+        cur = (int8_t)d;
+        dcount = 2;
+      }
+      prev = x;
+      push(x);
+      count = std::min(count + 1, 2u);
+    }
+  }
+  if (dcount >= 3 && dflag) {
+    byte((uint8_t)(dcount - 3));
+    byte((uint8_t)cur);
+    varint(first);
+    return out;
+  }
+  if (count == 1) {
+    if (lit == 127) {
+      close_group();
+      lit = 0;
+    }
+    if (lit == 0) {
+      lit_idx = out.size();
+      varint((T)1);
+    }
+    varint(buf[head ^ 1u]);
+    lit++;
+  }
+  if (count == 2) {
+    if (lit == 127) {
+      close_group();
+      lit = 0;
+    }
+    if (lit == 0) {
+      lit_idx = out.size();
+      byte(1);
+    }
+    varint(buf[head]);
+    lit++;
+    if (lit == 127) {
+      close_group();
+      lit = 0;
+    }
+    if (lit == 0) {
+      lit_idx = out.size();
+      byte(1);
+    }
+    head ^= 1u;
+    varint(buf[head]);
+    lit++;
+  }
+  close_group();
+  return out;
+}
+
+static std::string rlev1_ref_encode(const uint8_t *chunk, uint64_t n, uint32_t w) {
+  switch (w) {
+  case 1: return rlev1_ref_encode_chunk(chunk, n);
+  case 2: return rlev1_ref_encode_chunk(reinterpret_cast<const uint16_t *>(chunk), n);
+  case 4: return rlev1_ref_encode_chunk(reinterpret_cast<const uint32_t *>(chunk), n);
+  default: return rlev1_ref_encode_chunk(reinterpret_cast<const uint64_t *>(chunk), n);
+  }
 }
 
 // Assembles the RLE v1 container + side files from per-chunk streams.
@@ -361,6 +574,419 @@ static std::string rlev2_encode_chunk(const uint8_t *in, uint64_t n, uint32_t w)
   return s;
 }
 
+// The reference's RLE v2 chunk encoder, restated for byte-identical streams:
+// block_encode_orig (include/rlev2/encoder_trans.h:184-357) driving the
+// encoder of include/rlev2/encoder.h:83-591 (a port of ORC's RleEncoderV2)
+// with its dialect, all kept here: values are held as uint64 (READ_TYPE is
+// int64_t, utils.h:4-8), so nothing is ever negative; DELTA deltas are the
+// unsigned 64-bit differences (no absolute value); a block whose minimum is 0
+// and maximum at most 128 fails is_safe_subtract and is written DIRECT
+// (encoder.h:64-67); DIRECT widths are rounded to the aligned widths
+// 1, 2, 4, 8, 16, ...; a varying DELTA of width 1 is written with width 2;
+// a patch width of 64 becomes 56 with 8-bit base-reduced values
+// (encoder.h:205-211); histograms use uint8 counters (MAX_LITERAL_SIZE 128
+// never overflows them); the DIRECT/DELTA/PATCHED_BASE length field is
+// var_runlen - 1. One deviation, at the 128-zero run (see encode()).
+namespace v2ref {
+
+constexpr uint32_t kMaxLit = 128;  // utils.h:65 MAX_LITERAL_SIZE
+constexpr uint32_t kMinRep = 3;    // utils.h:66 MINIMUM_REPEAT
+constexpr uint32_t kMaxShort = 10; // utils.h:69 MAX_SHORT_REPEAT_LENGTH
+
+// utils.h:30-43 __BIT_WIDTH_ENCODE_MAP: width w -> 5-bit code
+inline uint8_t enc_width(uint32_t w) {
+  if (w <= 1) return 0;
+  if (w <= 24) return (uint8_t)(w - 1);
+  if (w <= 32) return (uint8_t)(24 + (w - 25) / 2);
+  return (uint8_t)(28 + (w - 33) / 8);
+}
+// utils.h:45-52 __CLOSEST_FIXED_BIT_MAP
+inline uint8_t closest(uint32_t b) {
+  if (b <= 1) return 1;
+  if (b <= 24) return (uint8_t)b;
+  if (b <= 32) return (uint8_t)((b + 1) & ~1u);
+  return (uint8_t)((b + 7) & ~7u);
+}
+// utils.h:54-62 __CLOSEST_ALIGNED_FIXED_BIT_MAP
+inline uint8_t closest_aligned(uint32_t b) {
+  if (b <= 1) return 1;
+  if (b == 2) return 2;
+  if (b <= 4) return 4;
+  if (b <= 8) return 8;
+  return (uint8_t)((b + 7) & ~7u);
+}
+inline uint8_t nbits(uint64_t v) { return closest(v ? 64 - __builtin_clzll(v) : 0); }
+
+struct Patch {
+  uint32_t len = 0, width = 0, gap_width = 0;
+  uint8_t bits95 = 0, bits100 = 0;
+  uint64_t min = 0;
+  uint64_t red[kMaxLit];
+  uint64_t gap_patch[kMaxLit];
+};
+
+struct Enc {
+  std::string out;
+  uint64_t deltas[kMaxLit] = {};
+  uint64_t lit[kMaxLit] = {};
+  uint32_t n = 0, fix = 0, var = 0;
+  uint8_t delta_bits = 0;
+  bool fixed_delta = false;
+
+  void put(uint8_t b) { out.push_back(static_cast<char>(b)); }
+
+  // encoder.h:83-120
+  void aligned(const uint64_t *in, uint32_t len, uint32_t bits) {
+    if (bits < 8) {
+      const uint64_t m = (1u << bits) - 1;
+      const uint32_t hops = 8 / bits, rem = len % hops, end = len - rem;
+      for (uint32_t i = 0; i < end; i += hops) {
+        uint8_t b = 0;
+        for (uint32_t j = 0; j < hops; j++) b |= (uint8_t)((in[i + j] & m) << (8 - (j + 1) * bits));
+        put(b);
+      }
+      if (rem) {
+        uint32_t sh = 8 - bits;
+        uint8_t b = 0;
+        for (uint32_t i = end; i < len; i++) {
+          b |= (uint8_t)((in[i] & m) << sh);
+          sh -= bits;
+        }
+        put(b);
+      }
+    } else {
+      const int32_t bytes = (int32_t)(bits / 8);
+      for (uint32_t i = 0; i < len; i++)
+        for (int32_t j = bytes - 1; j >= 0; j--) put((uint8_t)((in[i] >> (j * 8)) & 0xff));
+    }
+  }
+
+  // encoder.h:122-156
+  void unaligned(const uint64_t *in, uint32_t len, uint32_t bits) {
+    uint32_t left = 8;
+    uint8_t cur = 0;
+    for (uint32_t i = 0; i < len; i++) {
+      uint64_t v = in[i];
+      uint32_t todo = bits;
+      while (todo > left) {
+        cur |= (uint8_t)(v >> (todo - left));
+        todo -= left;
+        v &= (todo >= 64 ? ~0ull : ((1ull << todo) - 1));
+        put(cur);
+        cur = 0;
+        left = 8;
+      }
+      left -= todo;
+      cur |= (uint8_t)(v << left);
+      if (left == 0) {
+        put(cur);
+        cur = 0;
+        left = 8;
+      }
+    }
+    if (left != 8) put(cur);
+  }
+
+  // encoder.h:158-178: hist of width codes; width covering p percent
+  static void hist_of(const uint64_t *d, uint32_t len, uint8_t *h) {
+    for (int i = 0; i < 32; i++) h[i] = 0;
+    for (uint32_t i = 0; i < len; i++) h[enc_width(nbits(d[i]))]++;
+  }
+  static uint8_t pct_width(const uint8_t *h, uint32_t len, uint32_t p) {
+    p = std::min<uint32_t>(100 - p, 100);
+    int32_t plen = p == 0 ? 0 : (int32_t)((len * p - 1) / 100 + 1);
+    for (int i = 31; i >= 0; i--) {
+      plen -= h[i];
+      if (plen < 0) return kV2DecodeWidth[i];
+    }
+    return 0;
+  }
+
+  void short_repeat() {  // encoder.h:396-412
+    const uint64_t v = lit[0];
+    const uint32_t vb = (nbits(v) - 1) / 8 + 1;
+    put((uint8_t)(((vb - 1) << 3) | (fix - kMinRep)));
+    for (int32_t i = (int32_t)vb - 1; i >= 0; i--) put((uint8_t)((v >> (i * 8)) & 0xff));
+    fix = 0;
+    n = 0;
+    var = 0;
+  }
+
+  void direct() {  // encoder.h:424-440
+    uint8_t h[32];
+    hist_of(lit, n, h);
+    const uint32_t bits = closest_aligned(pct_width(h, n, 100));
+    const uint32_t len = var - 1;
+    put((uint8_t)(0x40 | (enc_width(bits) << 1) | ((len & 0x100) >> 8)));
+    put((uint8_t)(len & 0xff));
+    aligned(lit, n, bits);
+    var = 0;
+    n = 0;
+  }
+
+  void vulong(uint64_t v) {  // encoder.h:443-454
+    while (v & ~0x7full) {
+      put((uint8_t)(0x80 | (v & 0x7f)));
+      v >>= 7;
+    }
+    put((uint8_t)v);
+  }
+
+  void delta() {  // encoder.h:468-520
+    uint32_t len = 0;
+    uint32_t enc = 0;
+    uint32_t bits = closest_aligned(delta_bits);
+    if (fixed_delta) {
+      if (fix > kMinRep) {
+        len = fix - 1;
+        fix = 0;
+      } else {
+        len = var - 1;
+        var = 0;
+      }
+    } else {
+      if (bits == 1) bits = 2;
+      enc = enc_width(bits);
+      len = var - 1;
+      var = 0;
+    }
+    put((uint8_t)(0xC0 | (enc << 1) | ((len & 0x100) >> 8)));
+    put((uint8_t)(len & 0xff));
+    vulong(lit[0]);
+    const int64_t d0 = (int64_t)deltas[0];
+    vulong((uint64_t)((d0 << 1) ^ (d0 >> 63)));
+    if (!fixed_delta) aligned(deltas + 1, n - 2, bits);
+    for (uint32_t i = 0; i < n; i++) deltas[i] = 0;
+    n = 0;
+    var = 0;
+  }
+
+  void prepare_patch(Patch &pb) {  // encoder.h:196-266
+    uint64_t mask = (uint64_t)((int64_t)(1ull << pb.bits95) - 1);
+    pb.len = (n + 19) / 20;
+    pb.width = closest(pb.bits100 - pb.bits95);
+    if (pb.width == 64) {
+      pb.width = 56;
+      pb.bits95 = 8;
+      mask = (1ull << pb.bits95) - 1;
+    }
+    uint32_t gi = 0;
+    uint64_t gaps[kMaxLit], patches[kMaxLit];
+    uint64_t prev = 0, max_gap = 0;
+    for (uint32_t i = 0; i < n; i++) {
+      if (pb.red[i] > mask) {
+        const uint64_t gap = i - prev;
+        if (gap > max_gap) max_gap = gap;
+        prev = i;
+        gaps[gi] = gap;
+        patches[gi++] = pb.red[i] >> pb.bits95;
+        pb.red[i] &= mask;
+      }
+    }
+    pb.len = gi;
+    if (max_gap == 0 && pb.len != 0) pb.gap_width = 1;
+    else pb.gap_width = nbits(max_gap);
+    if (pb.gap_width > 8) {  // unreachable for 128-value blocks (gaps < 128)
+      pb.gap_width = 8;
+      pb.len += max_gap == 511 ? 2 : 1;
+    }
+    uint32_t pos = 0, k = 0;
+    for (uint32_t i = 0; i < pb.len; i++) {
+      uint64_t g = gaps[k], pv = patches[k];
+      k++;
+      while (g > 255) {
+        pb.gap_patch[pos++] = 255ull << pb.width;
+        i++;
+        g -= 255;
+      }
+      pb.gap_patch[pos++] = (g << pb.width) | pv;
+    }
+  }
+
+  void patched(Patch &pb) {  // encoder.h:522-580 (no value is negative)
+    var -= 1;
+    put((uint8_t)(0x80 | (enc_width(pb.bits95) << 1) | ((var & 0x100) >> 8)));
+    put((uint8_t)(var & 0xff));
+    const uint32_t vb = ((uint32_t)nbits(pb.min) + 1 - 1) / 8 + 1;
+    put((uint8_t)(((vb - 1) << 5) | enc_width(pb.width)));
+    put((uint8_t)(((pb.gap_width - 1) << 5) | pb.len));
+    for (int32_t i = (int32_t)vb - 1; i >= 0; i--) put((uint8_t)((pb.min >> (i * 8)) & 0xff));
+    unaligned(pb.red, n, closest(pb.bits95));
+    unaligned(pb.gap_patch, pb.len, closest(pb.gap_width + pb.width));
+    var = 0;
+    n = 0;
+  }
+
+  void determine() {  // encoder.h:268-394
+    if (n <= kMinRep) {
+      direct();
+      return;
+    }
+    bool inc = true, dec = true;
+    fixed_delta = true;
+    uint64_t lo = lit[0], hi = lit[0];
+    const uint64_t init = lit[1] - lit[0];
+    uint64_t maxd = 0;
+    for (uint32_t i = 1; i < n; i++) {
+      const uint64_t l1 = lit[i], l0 = lit[i - 1];
+      const uint64_t d = l1 - l0;
+      lo = std::min(lo, l1);
+      hi = std::max(hi, l1);
+      inc &= (l0 <= l1);
+      dec &= (l0 >= l1);
+      fixed_delta &= (d == init);
+      deltas[i - 1] = d;
+      maxd = std::max(maxd, d);
+    }
+    deltas[0] = lit[1] - lit[0];
+    // is_safe_subtract(hi, lo): lo > 0, or hi > lo - INT8_MIN (unsigned)
+    if (!(lo > 0 || hi > lo + 128)) {
+      direct();
+      return;
+    }
+    if (lo == hi || fixed_delta) {
+      delta();
+      return;
+    }
+    if (init != 0) {
+      delta_bits = nbits(maxd);
+      if (inc || dec) {
+        delta();
+        return;
+      }
+    }
+    uint8_t h[32];
+    hist_of(lit, n, h);
+    const uint32_t b100 = pct_width(h, n, 100), b90 = pct_width(h, n, 90);
+    if (b100 - b90 > 1) {
+      Patch pb;
+      pb.min = lo;
+      for (uint32_t i = 0; i < n; i++) pb.red[i] = lit[i] - lo;
+      hist_of(pb.red, n, h);
+      pb.bits95 = pct_width(h, n, 95);
+      pb.bits100 = pct_width(h, n, 100);
+      if (pb.bits100 - pb.bits95 != 0) {
+        prepare_patch(pb);
+        patched(pb);
+      } else {
+        direct();
+      }
+    } else {
+      direct();
+    }
+  }
+
+  // encoder_trans.h:229-339: one chunk's values
+  void encode(const uint64_t *vals, uint64_t count) {
+    uint64_t prev_delta = 0;
+    for (uint64_t r = 0; r < count; r++) {
+      const uint64_t val = vals[r];
+      if (n == 0) {
+        lit[n++] = val;
+        fix = 1;
+        var = 1;
+        continue;
+      }
+      if (n == 1) {
+        lit[n++] = val;
+        prev_delta = val - lit[0];
+        if (val == lit[0]) {
+          fix = 2;
+          var = 0;
+        } else {
+          fix = 0;
+          var = 2;
+        }
+        continue;
+      }
+      const uint64_t cur = val - lit[n - 1];
+      if (prev_delta == 0 && cur == 0) {
+        lit[n++] = val;
+        if (var > 0) fix = 2;
+        fix++;
+        if (fix >= kMinRep && var > 0) {
+          n -= kMinRep;
+          var -= kMinRep - 1;
+          determine();
+          prev_delta = 0;
+          for (uint32_t ii = 0; ii < kMinRep; ii++) {
+            lit[ii] = val;
+            deltas[ii] = 0;
+          }
+          n = kMinRep;
+        } else if (fix >= kMinRep) {
+          deltas[0] = 0;
+        }
+        if (fix == kMaxLit) {
+          // encoder_trans.h:289-291 calls determineEncoding here with
+          // var_runlen 0. For a run of 128 zeros is_safe_subtract(0, 0)
+          // fails, so the reference writes DIRECT with length field
+          // var_runlen - 1 = 511 ahead of 128 values: a stream its own
+          // decoder cannot read. The one deviation from the reference's
+          // bytes: that run is written as the fixed DELTA block determine()
+          // writes for a run of any other value.
+          if (lit[0] == 0) {
+            fixed_delta = true;
+            delta();
+          } else {
+            determine();
+          }
+        }
+        continue;
+      }
+      if (fix >= kMinRep) {
+        if (fix <= kMaxShort) {
+          short_repeat();
+        } else {
+          fixed_delta = true;
+          delta();
+        }
+      }
+      if (fix > 0 && fix < kMinRep && val != lit[n - 1]) {
+        var = fix;
+        fix = 0;
+      }
+      if (n == 0) {
+        lit[n++] = val;
+        fix = 1;
+        var = 1;
+      } else {
+        prev_delta = val - lit[n - 1];
+        lit[n++] = val;
+        var++;
+        if (var == kMaxLit) determine();
+      }
+    }
+    if (n != 0) {
+      if (var != 0) {
+        determine();
+      } else if (fix != 0) {
+        if (fix < kMinRep) {
+          var = fix;
+          fix = 0;
+          determine();
+        } else if (fix <= kMaxShort) {
+          short_repeat();
+        } else {
+          fixed_delta = true;
+          delta();
+        }
+      }
+    }
+  }
+};
+
+}  // namespace v2ref
+
+// One chunk of a w-byte column (the values [0, count)), reference choices.
+static std::string rlev2_ref_encode(const uint8_t *in, uint64_t count, uint32_t w) {
+  std::vector<uint64_t> v(count);
+  for (uint64_t i = 0; i < count; i++) v[i] = load_le(in + i * w, w);
+  auto e = std::make_unique<v2ref::Enc>();
+  e->encode(v.data(), count);
+  return std::move(e->out);
+}
+
 static codag_status rlev2_assemble(const std::vector<std::string> &streams, uint64_t raw_bytes,
                                    uint8_t **comp, uint64_t *comp_bytes, uint64_t **blk_off,
                                    uint64_t **col_len) {
@@ -411,10 +1037,12 @@ codag_status codag_rlev1_encode(const uint8_t *in, uint64_t in_bytes, uint32_t w
   if (nc == 0) nc = 32;
   std::vector<std::string> streams(nc);
   run_threads(nc, 0, [&](uint64_t i) {
-    std::vector<uint8_t> buf(chunk_bytes, 0);  // zero padding (rlev1.h:127)
+    // zero padding (rlev1.h:127); 8-byte aligned so the column reads as T
+    std::vector<uint64_t> buf((chunk_bytes + 7) / 8, 0);
+    uint8_t *b = reinterpret_cast<uint8_t *>(buf.data());
     uint64_t lo = i * chunk_bytes;
-    if (lo < in_bytes) std::memcpy(buf.data(), in + lo, std::min(chunk_bytes, in_bytes - lo));
-    streams[i] = rlev1_encode_chunk(buf.data(), chunk_bytes / w, w);
+    if (lo < in_bytes) std::memcpy(b, in + lo, std::min(chunk_bytes, in_bytes - lo));
+    streams[i] = rlev1_ref_encode(b, chunk_bytes / w, w);
   });
   if (n_chunks) *n_chunks = nc;
   return rlev1_assemble(streams, in_bytes, container, container_bytes, blk_off, blk_off_entries,
@@ -432,10 +1060,12 @@ codag_status codag_rlev2_encode(const uint8_t *in, uint64_t in_bytes, uint32_t w
   if (nc == 0) nc = 32;
   std::vector<std::string> streams(nc);
   run_threads(nc, 0, [&](uint64_t i) {
-    std::vector<uint8_t> buf(chunk_bytes, 0);
-    uint64_t lo = i * chunk_bytes;
-    if (lo < in_bytes) std::memcpy(buf.data(), in + lo, std::min(chunk_bytes, in_bytes - lo));
-    streams[i] = rlev2_encode_chunk(buf.data(), chunk_bytes / w, w);
+    // encoder_trans.h:207-209: chunk i encodes the whole values of
+    // [i*chunk, min((i+1)*chunk, in_bytes)); no padding values
+    const uint64_t lo = i * chunk_bytes;
+    const uint64_t hi = std::min((i + 1) * chunk_bytes, in_bytes);
+    const uint64_t count = hi > lo ? (hi - lo) / w : 0;
+    streams[i] = rlev2_ref_encode(in + lo, count, w);
   });
   if (n_chunks) *n_chunks = nc;
   return rlev2_assemble(streams, in_bytes, comp, comp_bytes, blk_off, col_len);

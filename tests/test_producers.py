@@ -50,6 +50,11 @@ def test_rlev1_encode_round_trip(width):
 @pytest.mark.parametrize("width", [1, 2, 4, 8])
 def test_rlev2_encode_round_trip(width):
     col = _column(width, 2 * CH // width + 777, 10 + width)
+    if width == 8:
+        # values below 2^32: the reference encoder writes PATCHED_BASE
+        # patches up to 56 bits wide, which its decoder masks to 32 bits
+        # (decoder_new.h:1105), see test_rlev2_wide_patch_is_masked_as_the_reference_decodes
+        col = col & np.uint64(0xFFFFFFFF)
     comp, blk, cl, n = ca.rlev2_encode(col, width)
     assert n % 32 == 0 and blk[n] == col.nbytes and np.all(blk[:n] % 128 == 0)
     st, out = oracle.rlev2_decode_chunks(comp, blk[:n], cl, width, CH, threads=4)
@@ -138,3 +143,24 @@ def test_deflate_mixed_blocks():
     s, counts = ca.deflate_compress_mixed(raw, 20, 5, 16384, seed=2)
     assert sum(counts) > 20 and counts[1] > counts[0] > 0
     assert zlib.decompress(s.tobytes()) == raw.tobytes()
+
+
+def test_rlev2_wide_patch_is_masked_as_the_reference_decodes():
+    # A uint64 block of small values and one outlier: the reference encoder
+    # (encoder.h:196-266) patches it with a 56-bit patch (header byte 3 =
+    # 0x1e: one base byte, patch width code 30); the reference decoder keeps
+    # only the low 32 bits of a patch (uint32 patch mask, decoder_new.h:1105),
+    # and so does the oracle. Byte identity with the reference therefore
+    # means this column does not round-trip, in the reference either.
+    small = [121, 160, 96, 122, 150, 173, 117, 101, 197, 193, 36, 178, 92, 5, 16, 180, 77, 27,
+             180, 72, 11, 17, 45, 145, 132, 21, 153, 178, 101, 74, 192, 163, 167, 75, 162, 104,
+             79, 137, 176, 6, 110, 5]
+    big = 237880511751946841
+    col = np.array(small + [big], np.uint64)
+    comp, blk, cl, n = ca.rlev2_encode(col, 8)
+    s = comp[: int(cl[0])].tobytes()
+    assert s[:4] == bytes.fromhex("8e2a1ea1")  # PATCHED_BASE, 8-bit values, 56-bit patch
+    st, dec = oracle.rlev2_decode_chunk(s, 8, CH // 8)
+    assert st == 0 and dec[:42].tolist() == small
+    red = big - min(small)
+    assert int(dec[42]) == min(small) + ((red & 0xFF) | (((red >> 8) & 0xFFFFFFFF) << 8))
