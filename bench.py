@@ -1,20 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X CODAG decoders (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c5]
 
-A step is one pass of the hot path over the whole synthetic workload, with the
-compressed input already resident in HBM. The default workload is ORC RLE v2
-decode of a uint32 and a uint64 column of 2^26 values each (128 KiB chunks).
-Before timing, the GPU output is checked bit-exact against the raw column
-(byte compare + CRC32). Prints ONE JSON line.
+Prints ONE JSON line. With the default ``--config all`` the line is headlined
+on the largest single-GPU workload -- the batched uneven Deflate decode of
+10,000 zlib streams (4 KiB..4 MiB) -- and carries the other four workloads
+under ``config.configs``:
 
-Roofline: the dominant kernel's algorithmic bytes per launch (compressed bytes
-read + decoded bytes written + blk_off/col_len metadata) divided by its mean
-launch time, measured with HIP events on the stream the kernel runs on,
-against MI355X's 8 TB/s HBM peak. cpu_baseline: the CPU restatement of the
-same decoder (or system zlib for Deflate), multithreaded over chunks on the
-host cores, timed on a bounded sample of the same workload.
+* c1  ORC RLE v1 decode of a 2^24-value uint64 column (128 MiB), 128 KiB chunks
+* c2  ORC RLE v2 decode of a uint32 and a uint64 column, 2^26 values each
+* c3  Deflate decode of 1 GiB in 8192 zlib level-6 chunks of 128 KiB
+* c4  batched uneven Deflate (the headline)
+* c5  served mixed-codec requests, Poisson arrivals, p50/p99 latency
+
+A step is one pass of the hot path over the whole workload with the
+compressed input already resident in HBM. Every workload's GPU output is
+checked bit-exact (byte compare + CRC32) against the raw data before the
+warm-up and again after the timed steps.
+
+Roofline: the dominant kernel's algorithmic bytes per launch (compressed
+bytes read + decoded bytes written + offset/length metadata) divided by its
+mean launch time, measured with HIP events on the stream the kernel runs on,
+against MI355X's 8 TB/s HBM peak. ``traffic`` is the HBM bytes per launch that
+rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE measured for the same kernel and
+workload (profiles/traffic.json, scaled by a plain copy kernel).
+
+cpu_baseline: the CPU restatement of the same decoder (system zlib inflate
+for Deflate), multithreaded over chunks/streams on every core this process
+may run on, decoding RAM to preallocated RAM; also a single-thread figure.
 """
 from __future__ import annotations
 
@@ -41,7 +55,22 @@ def log(msg):
 
 
 def host_threads() -> int:
-    return max(1, min(16, os.cpu_count() or 1))
+    """Every core this process may run on."""
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def crc32(b: np.ndarray) -> int:
@@ -52,7 +81,7 @@ def crc32(b: np.ndarray) -> int:
 def traffic_from_profiles(kernel_key: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any.
 
-    Keys are "<kernel>:<config>:<workgroups>"; the bytes come from FETCH_SIZE
+    Keys are "<kernel>:<workload>:<workgroups>"; the bytes come from FETCH_SIZE
     and WRITE_SIZE scaled by the same counters of the plain copy kernel."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -62,6 +91,45 @@ def traffic_from_profiles(kernel_key: str):
     except (OSError, ValueError):
         return None
 
+
+def roofline(kernel, kernel_ms, alg_bytes, traffic_key):
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+    tr = traffic_from_profiles(traffic_key)
+    r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr, "kernel": kernel,
+         "kernel_ms": round(kernel_ms, 4), "alg_bytes_per_launch": int(alg_bytes)}
+    if tr:
+        r["traffic_over_alg"] = round(tr / alg_bytes, 3)
+    return r
+
+
+def time_cpu(fn, min_runs=5, budget_s=2.0):
+    """Median wall time of fn() after one warm-up run: at least min_runs runs,
+    more while the budget lasts."""
+    fn()
+    times = []
+    t0 = time.perf_counter()
+    while len(times) < min_runs or (time.perf_counter() - t0 < budget_s and len(times) < 50):
+        t1 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t1)
+    return float(np.median(times)), len(times)
+
+
+def cpu_leg(multi, multi_bytes, multi_desc, single, single_bytes, single_desc, what):
+    """cpu_baseline object: `multi` on every core, `single` on one thread."""
+    threads = host_threads()
+    tm, nm = time_cpu(multi)
+    ts, ns = time_cpu(single, min_runs=5, budget_s=1.0)
+    return {"value": round(multi_bytes / tm / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": "port", "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "single_thread_gbs": round(single_bytes / ts / 1e9, 4),
+            "sample": f"{multi_desc}, {what} on {threads} threads (all cores of this process), "
+                      f"median of {nm} runs after a warm-up; single thread: {single_desc}, "
+                      f"median of {ns}"}
+
+
+# ----------------------------------------------------------------- RLE (c1, c2)
 
 class Column:
     """One device-resident RLE container."""
@@ -88,6 +156,17 @@ class Column:
         t = self.tune
         return None if t is None else [t.threads_per_group, t.window_bytes, t.queue_depth]
 
+    def kernel(self, ctx):
+        """Name of the kernel the column's launch shape selects."""
+        t = self.tune or ctx.tune_get(self.codec)
+        if self.codec == ca.RLEV1:
+            if t.queue_depth == 4 and t.window_bytes == 2048:
+                return "rlev1_pipe_kernel"
+            return "rlev1_pc_kernel" if t.queue_depth == 2 else "rlev1_decode_kernel"
+        if t.queue_depth == 2 and t.threads_per_group in (128, 192, 256):
+            return "rlev2_pc_kernel"
+        return "rlev2_wave_kernel" if t.threads_per_group == 64 else "rlev2_decode_kernel"
+
     def launch(self, ctx):
         if self.tune is not None:  # host-side parameter switch, no device work
             ctx.tune_set(self.codec, self.tune)
@@ -95,7 +174,7 @@ class Column:
         fn(self.d_comp, self.comp_bytes, self.d_off, self.d_len, self.n, self.width, CH,
            self.d_out, self.d_st)
 
-    def verify(self, ctx):
+    def verify(self):
         st = self.d_st.download(4 * self.n, np.int32)
         out = self.d_out.download(len(self.raw))
         ok = bool(np.all(st == 0)) and out.tobytes() == self.raw.tobytes()
@@ -118,51 +197,52 @@ def make_rle(ctx, cfg):
     return cols, desc, dtype
 
 
-def cpu_rle_baseline(cols, budget_s=10.0):
+def cpu_rle_baseline(cols):
     import oracle
+
     threads = host_threads()
-    col = max(cols, key=lambda c: c.n * CH)
-    comp, blk, cl = col.host
-    fn = oracle.rlev1_decode_chunks if col.codec == ca.RLEV1 else oracle.rlev2_decode_chunks
-    k = min(col.n, 256)
-    sel = slice(0, k)
-    fn(comp, blk[sel], cl[sel], col.width, CH, threads)  # warm-up
-    t0 = time.perf_counter()
-    reps = 0
-    times = []
-    while time.perf_counter() - t0 < budget_s and reps < 50:
-        t1 = time.perf_counter()
-        fn(comp, blk[sel], cl[sel], col.width, CH, threads)
-        times.append(time.perf_counter() - t1)
-        reps += 1
-    med = float(np.median(times))
-    return {"value": round(k * CH / med / 1e9, 3), "unit": "GB/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{k} chunks ({k * CH >> 20} MiB out) of the {col.width * 8}-bit column, "
-                      f"median of {reps} runs, oracle C decoder on {threads} threads"}
+    outs = [np.empty(c.n * CH, np.uint8) for c in cols]
+
+    def run(th, k=None):
+        for c, o in zip(cols, outs):
+            comp, blk, cl = c.host
+            fn = oracle.rlev1_decode_chunks if c.codec == ca.RLEV1 else oracle.rlev2_decode_chunks
+            m = c.n if k is None else min(k, c.n)
+            fn(comp, blk[:m], cl[:m], c.width, CH, th, out=o)
+
+    total = sum(c.n * CH for c in cols)
+    k1 = 64
+    single = sum(min(k1, c.n) * CH for c in cols)
+    return cpu_leg(lambda: run(threads), total, f"the whole workload ({total >> 20} MiB out)",
+                   lambda: run(1, k1), single,
+                   f"the first {k1} chunks of each column ({single >> 20} MiB out)",
+                   "oracle C decoder")
 
 
 def bench_rle(ctx, args, cfg):
-    log(f"[bench] generating workload {cfg}")
+    log(f"[bench] {cfg}: generating")
     cols, desc, dtype = make_rle(ctx, cfg)
     out_bytes = sum(len(c.raw) for c in cols)
     if not args.no_autotune:
         for c in cols:
             ms = c.autotune(ctx)
-            log(f"[bench] autotuner: {c.width * 8}-bit column -> launch shape {c.shape()} "
+            log(f"[bench] {cfg}: autotuner: {c.width * 8}-bit column -> launch shape {c.shape()} "
                 f"({ms:.3f} ms)")
+
+    def check():
+        crcs = []
+        for c in cols:
+            ok, crc = c.verify()
+            if not ok or crc != crc32(c.raw):
+                raise SystemExit(f"{cfg}: bit-exact check failed for codec {c.codec}/{c.width}")
+            crcs.append(crc)
+        return crcs
+
     for c in cols:
         c.launch(ctx)
     ctx.sync()
-    crcs = []
-    for c in cols:
-        ok, crc = c.verify(ctx)
-        if not ok:
-            raise SystemExit(f"bit-exact check failed for {c.codec}/{c.width}")
-        crcs.append(crc)
-        if crc != crc32(c.raw):
-            raise SystemExit("CRC32 mismatch")
-    log("[bench] bit-exact check passed")
+    crcs = check()
+    log(f"[bench] {cfg}: bit-exact check passed")
     for _ in range(args.warmup):
         for c in cols:
             c.launch(ctx)
@@ -170,18 +250,15 @@ def bench_rle(ctx, args, cfg):
     # timed region: events around every launch on the context stream
     dom = max(range(len(cols)), key=lambda i: cols[i].alg_bytes)
     per = [0.0] * len(cols)
-    ctx.sync()
-    t_wall = time.perf_counter()
     ctx.event(0)
     slot = 1
     marks = []
     for s in range(args.steps):
         for i, c in enumerate(cols):
-            a = slot
-            ctx.event(a)
+            ctx.event(slot)
             c.launch(ctx)
-            ctx.event(a + 1)
-            marks.append((i, a, a + 1))
+            ctx.event(slot + 1)
+            marks.append((i, slot, slot + 1))
             slot += 2
             if slot >= 250:  # drain event slots
                 for (j, x, y) in marks:
@@ -191,51 +268,29 @@ def bench_rle(ctx, args, cfg):
     ctx.event(255)
     total_ms = ctx.elapsed(0, 255)
     ctx.sync()
-    wall = time.perf_counter() - t_wall
     for (j, x, y) in marks:
         per[j] += ctx.elapsed(x, y)
+    if check() != crcs:
+        raise SystemExit(f"{cfg}: output changed during the timed steps")
     ms_step = total_ms / args.steps
-    gbs = out_bytes / (ms_step / 1e3) / 1e9
     k_ms = per[dom] / args.steps
-    achieved = cols[dom].alg_bytes / (k_ms / 1e3) / 1e9
-    if cols[dom].codec == ca.RLEV1:  # the launch shape selects the kernel
-        t = cols[dom].tune or ctx.tune_get(ca.RLEV1)
-        if t.queue_depth == 4 and t.window_bytes == 2048:
-            kname = "rlev1_pipe_kernel"
-        elif t.queue_depth == 2:
-            kname = "rlev1_pc_kernel"
-        else:
-            kname = "rlev1_decode_kernel"
-    else:  # the launch shape selects the kernel
-        t = cols[dom].tune or ctx.tune_get(ca.RLEV2)
-        if t.queue_depth == 2 and t.threads_per_group in (128, 256):
-            kname = "rlev2_pc_kernel"
-        elif t.threads_per_group == 64:
-            kname = "rlev2_wave_kernel"
-        else:
-            kname = "rlev2_decode_kernel"
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(f"{kname}:{cfg}:{cols[dom].n}"),
-            "kernel": kname, "kernel_ms": round(k_ms, 4),
-            "alg_bytes_per_launch": int(cols[dom].alg_bytes)}
-    cpu = None if args.no_cpu else cpu_rle_baseline(cols)
+    kname = cols[dom].kernel(ctx)
     return {
-        "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
-        "data": "synthetic", "bit_exact": True,
-        "config": {"workload": desc, "chunks": [c.n for c in cols],
-                   "compressed_bytes": [c.comp_bytes for c in cols],
-                   "decompressed_bytes": out_bytes, "crc32": crcs,
-                   "wall_s": round(wall, 3),
-                   "launch_shapes": [c.shape() for c in cols]},
-        "roofline": roof, "cpu_baseline": cpu,
+        "workload": desc, "value": round(out_bytes / (ms_step / 1e3) / 1e9, 2), "unit": "GB/s",
+        "ms_per_step": round(ms_step, 4), "dtype": dtype, "bit_exact": True,
+        "chunks": [c.n for c in cols], "compressed_bytes": [c.comp_bytes for c in cols],
+        "decompressed_bytes": out_bytes, "crc32": crcs,
+        "launch_shapes": [c.shape() for c in cols],
+        "kernel_ms_per_column": [round(p / args.steps, 4) for p in per],
+        "roofline": roofline(kname, k_ms, cols[dom].alg_bytes, f"{kname}:{cfg}:{cols[dom].n}"),
+        "cpu_baseline": None if args.no_cpu else cpu_rle_baseline(cols),
     }
 
 
+# ---------------------------------------------------------------- Deflate (c3)
+
 def bench_deflate(ctx, args):
-    log("[bench] generating 1 GiB Deflate workload")
+    log("[bench] c3: generating 1 GiB Deflate workload")
     n_bytes = 1 << 30
     raw = ca.synth_deflate_payload(n_bytes, CH, seed=0xC0DA6 + 3)
     comp, blk, cl, n = ca.deflate_compress_chunks(raw, CH, level=6, threads=host_threads())
@@ -249,15 +304,17 @@ def bench_deflate(ctx, args):
     def launch():
         ctx.inflate_chunks(d_comp, len(comp), d_off, n, CH, d_out, d_len, d_st, 0)
 
+    def check():
+        st = d_st.download(4 * n, np.int32)
+        out = d_out.download(len(raw))
+        if not (np.all(st == 0) and out.tobytes() == raw.tobytes()):
+            raise SystemExit("c3: Deflate bit-exact check failed")
+        return crc32(out)
+
     launch()
     ctx.sync()
-    st = d_st.download(4 * n, np.int32)
-    out = d_out.download(len(raw))
-    if not (np.all(st == 0) and out.tobytes() == raw.tobytes()):
-        raise SystemExit("Deflate bit-exact check failed")
-    crc = crc32(out)
-    del out
-    log("[bench] bit-exact check passed")
+    crc = check()
+    log("[bench] c3: bit-exact check passed")
     for _ in range(args.warmup):
         launch()
     ctx.sync()
@@ -266,43 +323,31 @@ def bench_deflate(ctx, args):
         launch()
     ctx.event(1)
     ms = ctx.elapsed(0, 1) / args.steps
-    gbs = n_bytes / (ms / 1e3) / 1e9
+    if check() != crc:
+        raise SystemExit("c3: output changed during the timed steps")
     alg = len(comp) + n * CH + 8 * (n + 1)
-    achieved = alg / (ms / 1e3) / 1e9
     cpu = None
     if not args.no_cpu:
         import oracle
-        threads = host_threads()
-        k = min(n, 2048)
-        sub = off[: k + 1]
-        oracle.inflate_chunks(comp, sub, CH, threads)
-        times = []
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < 10 and len(times) < 20:
-            t1 = time.perf_counter()
-            oracle.inflate_chunks(comp, sub, CH, threads)
-            times.append(time.perf_counter() - t1)
-        med = float(np.median(times))
-        cpu = {"value": round(k * CH / med / 1e9, 3), "unit": "GB/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{k} chunks ({k * CH >> 20} MiB out), system zlib {oracle.zlib_version()} "
-                         f"inflate on {threads} threads, median of {len(times)} runs"}
+        out_h = np.empty(n * CH, np.uint8)
+        k1 = 256
+        cpu = cpu_leg(lambda: oracle.inflate_chunks(comp, off, CH, host_threads(), out=out_h),
+                      n * CH, f"the whole workload ({n} chunks, {n * CH >> 20} MiB out)",
+                      lambda: oracle.inflate_chunks(comp, off[: k1 + 1], CH, 1, out=out_h),
+                      k1 * CH, f"the first {k1} chunks ({k1 * CH >> 20} MiB out)",
+                      f"system zlib {oracle.zlib_version()} inflate")
     return {
-        "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
-        "data": "synthetic", "bit_exact": True,
-        "config": {"workload": "deflate 1 GiB, 8192 x 128 KiB zlib level-6 chunks "
-                               "(50% order-2 Markov text, 50% Zipf(1.1) int32)",
-                   "compressed_bytes": int(len(comp)), "crc32": crc},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(f"inflate_chunks_kernel:c3:{n}"),
-                     "kernel": "inflate_chunks_kernel", "kernel_ms": round(ms, 4),
-                     "alg_bytes_per_launch": int(alg)},
+        "workload": "deflate 1 GiB, 8192 x 128 KiB zlib level-6 chunks "
+                    "(50% order-2 Markov text, 50% Zipf(1.1) int32)",
+        "value": round(n_bytes / (ms / 1e3) / 1e9, 2), "unit": "GB/s", "ms_per_step": round(ms, 4),
+        "dtype": "u8", "bit_exact": True, "chunks": int(n), "compressed_bytes": int(len(comp)),
+        "decompressed_bytes": n_bytes, "crc32": crc,
+        "roofline": roofline("inflate_chunks_kernel", ms, alg, f"inflate_chunks_kernel:c3:{n}"),
         "cpu_baseline": cpu,
     }
 
+
+# ----------------------------------------------------------- batched (c4)
 
 def _parallel(fn, items):
     from concurrent.futures import ThreadPoolExecutor
@@ -313,61 +358,62 @@ def _parallel(fn, items):
 def bench_batch(ctx, args):
     """Uneven batch: n zlib streams, sizes log-uniform 4 KiB..4 MiB, blocks mixed
     fixed/dynamic/stored 20/75/5, decoded by one persistent graph-replayed launch."""
-    import ctypes
     rng = np.random.default_rng(0xC0DA6 + 4)
     n = args.streams
     sizes = np.exp(rng.uniform(np.log(4096), np.log(4 << 20), n)).astype(np.int64)
     total = int(sizes.sum())
-    log(f"[bench] generating {n} streams, {total >> 20} MiB")
+    log(f"[bench] c4: generating {n} streams, {total >> 20} MiB")
     payload = ca.synth_deflate_payload(total, CH, seed=0xC0DA6 + 4)
     poff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     res = _parallel(lambda i: ca.deflate_compress_mixed(payload[poff[i]:poff[i + 1]], 20, 5, 32768,
                                                         seed=i), range(n))
-    streams = [r[0] for r in res]
     mix = np.sum([r[1] for r in res], axis=0)
     order = np.argsort(-sizes, kind="stable")  # largest first: the queue balances the tail
-    slen = np.array([len(streams[i]) for i in order], np.int64)
+    slen = np.array([len(res[i][0]) for i in order], np.int64)
     soff = np.concatenate([[0], np.cumsum(slen)]).astype(np.int64)
     cap = sizes[order]
     doff = np.concatenate([[0], np.cumsum(cap)]).astype(np.int64)
-    d_src = ctx.upload(np.concatenate([streams[i] for i in order]))
-    del streams, res
+    src_h = np.concatenate([res[i][0] for i in order])
+    del res
+    d_src = ctx.upload(src_h)
     d_dst = ctx.alloc(int(doff[-1]))
     descs = (ca.StreamDesc * n)()
     for k in range(n):
         descs[k].src, descs[k].src_bytes = d_src.offset(int(soff[k])), int(slen[k])
         descs[k].dst, descs[k].dst_cap = d_dst.offset(int(doff[k])), int(cap[k])
     d_descs = ctx.upload(np.frombuffer(descs, np.uint8))
-    rsz = ctypes.sizeof(ca.StreamResult)
+    rsz = 16
     d_res = ctx.alloc(n * rsz)
-
-    # persistent launch: the context's waves per CU (a launch is min(CUs x
-    # waves per CU, streams) waves; 8..28 per CU time within noise here)
     best = ctx.tune_get(ca.DEFLATE)
-    log(f"[bench] persistent launch: {best.groups_per_cu} waves per CU")
+    log(f"[bench] c4: persistent launch, {best.groups_per_cu} waves per CU")
+
+    def check():
+        rr = np.frombuffer(d_res.download(n * rsz).tobytes(), dtype=np.dtype(
+            [("bytes", "<u8"), ("status", "<i4"), ("adler", "<u4")]))
+        out = d_dst.download(int(doff[-1]))
+        for k in range(n):
+            i = order[k]
+            if rr["status"][k] != 0 or int(rr["bytes"][k]) != sizes[i] or \
+                    not np.array_equal(out[doff[k]:doff[k + 1]], payload[poff[i]:poff[i + 1]]):
+                raise SystemExit(f"c4: batch bit-exact check failed at stream {i}")
+        return crc32(out)
 
     ctx.inflate_batch(d_descs, n, d_res)
     ctx.sync()
-    rr = np.frombuffer(d_res.download(n * rsz).tobytes(), dtype=np.dtype(
-        [("bytes", "<u8"), ("status", "<i4"), ("adler", "<u4")]))
-    out = d_dst.download(int(doff[-1]))
-    for k in range(n):
-        i = order[k]
-        if rr["status"][k] != 0 or int(rr["bytes"][k]) != sizes[i] or \
-                out[doff[k]:doff[k + 1]].tobytes() != payload[poff[i]:poff[i + 1]].tobytes():
-            raise SystemExit(f"batch bit-exact check failed at stream {i}")
-    crc = crc32(out)
-    del out
-    log("[bench] bit-exact check passed")
+    crc = check()
+    log("[bench] c4: bit-exact check passed")
     for _ in range(args.warmup):
         ctx.inflate_batch(d_descs, n, d_res)
     ctx.sync()
+    t_wall = time.perf_counter()
     ctx.event(0)
     for _ in range(args.steps):
         ctx.inflate_batch(d_descs, n, d_res)
     ctx.event(1)
     ms = ctx.elapsed(0, 1) / args.steps
-    gbs = total / (ms / 1e3) / 1e9
+    wall = time.perf_counter() - t_wall
+    if check() != crc:
+        raise SystemExit("c4: output changed during the timed steps")
 
     # plain per-file launches (one inflate call per stream) on a seeded sample
     samp = np.sort(np.random.default_rng(5).choice(n, size=min(n, 64), replace=False))
@@ -389,65 +435,57 @@ def bench_batch(ctx, args):
     pf_ms = ctx.elapsed(2, 3)
     pf_gbs = float(cap[samp].sum()) / (pf_ms / 1e3) / 1e9
     if not np.all(d_st1.download(4 * len(samp), np.int32) == 0):
-        raise SystemExit("per-file launch failed")
+        raise SystemExit("c4: per-file launch failed")
 
     alg = int(slen.sum()) + total + n * (32 + 16)
     cpu = None
     if not args.no_cpu:
         import oracle
-        threads = host_threads()
+        so = soff[:-1].astype(np.uint64)
+        out_h = np.empty(total, np.uint8)
         cs = samp[:64]
-        src_h = np.concatenate([d_src.download(int(soff[-1]))[soff[k]:soff[k] + slen[k]] for k in cs])
-        so = np.concatenate([[0], np.cumsum(slen[cs])[:-1]]).astype(np.uint64)
-        do = np.concatenate([[0], np.cumsum(cap[cs])[:-1]]).astype(np.uint64)
-        args_c = (src_h, so, slen[cs], do, cap[cs], int(cap[cs].sum()), threads)
-        oracle.inflate_batch(*args_c)
-        times = []
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < 10 and len(times) < 20:
-            t1 = time.perf_counter()
-            oracle.inflate_batch(*args_c)
-            times.append(time.perf_counter() - t1)
-        med = float(np.median(times))
-        cpu = {"value": round(float(cap[cs].sum()) / med / 1e9, 3), "unit": "GB/s",
-               "cores": threads, "kind": "port",
-               "sample": f"{len(cs)} of the streams ({int(cap[cs].sum()) >> 20} MiB out), system zlib "
-                         f"{oracle.zlib_version()} inflate on {threads} threads, median of {len(times)}"}
+        src_s = np.concatenate([src_h[soff[k]:soff[k] + slen[k]] for k in cs])
+        so_s = np.concatenate([[0], np.cumsum(slen[cs])[:-1]]).astype(np.uint64)
+        do_s = np.concatenate([[0], np.cumsum(cap[cs])[:-1]]).astype(np.uint64)
+        s_bytes = int(cap[cs].sum())
+        cpu = cpu_leg(lambda: oracle.inflate_batch(src_h, so, slen, doff[:-1], cap, total,
+                                                   host_threads(), out=out_h),
+                      total, f"the whole batch ({n} streams, {total >> 20} MiB out)",
+                      lambda: oracle.inflate_batch(src_s, so_s, slen[cs], do_s, cap[cs], s_bytes, 1,
+                                                   out=out_h),
+                      s_bytes, f"{len(cs)} of the streams ({s_bytes >> 20} MiB out)",
+                      f"system zlib {oracle.zlib_version()} inflate, one stream per task")
     return {
-        "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
-        "data": "synthetic", "bit_exact": True,
-        "config": {"workload": f"batched uneven deflate, {n} zlib streams log-uniform 4 KiB-4 MiB, "
-                               "block mix fixed/dynamic/stored 20/75/5, one persistent "
-                               "graph-replayed launch",
-                   "decompressed_bytes": total, "compressed_bytes": int(slen.sum()),
-                   "blocks_fixed_dynamic_stored": [int(x) for x in mix], "crc32": crc,
-                   "groups_per_cu": int(best.groups_per_cu),
-                   "per_file_launch_gbs": round(pf_gbs, 3),
-                   "per_file_sample": f"{len(samp)} streams, one inflate launch each"},
-        "roofline": {"bound": "hbm", "achieved": round(alg / (ms / 1e3) / 1e9, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("inflate_batch_kernel:c4"),
-                     "kernel": "inflate_batch_kernel", "kernel_ms": round(ms, 4),
-                     "alg_bytes_per_launch": alg},
+        "workload": f"batched uneven deflate, {n} zlib streams log-uniform 4 KiB-4 MiB, "
+                    "block mix fixed/dynamic/stored 20/75/5, one persistent graph-replayed launch",
+        "value": round(total / (ms / 1e3) / 1e9, 2), "unit": "GB/s", "ms_per_step": round(ms, 4),
+        "dtype": "u8", "bit_exact": True, "streams": n,
+        "decompressed_bytes": total, "compressed_bytes": int(slen.sum()),
+        "blocks_fixed_dynamic_stored": [int(x) for x in mix], "crc32": crc,
+        "groups_per_cu": int(best.groups_per_cu), "wall_s": round(wall, 3),
+        "per_file_launch_gbs": round(pf_gbs, 3),
+        "per_file_sample": f"{len(samp)} streams, one inflate launch each",
+        "roofline": roofline("inflate_batch_kernel", ms, alg, "inflate_batch_kernel:c4"),
         "cpu_baseline": cpu,
     }
 
 
+# ------------------------------------------------------------- served (c5)
+
 def bench_served(ctx, args):
     """Served mix (40% RLE v1 u64, 30% RLE v2 u32, 30% Deflate; sizes log-uniform
     64 KiB..256 MiB) through the continuous-batching server, device-resident,
-    200 GB admission budget; Poisson arrivals at 10..90% of the measured peak."""
+    200 GB admission budget. Arrivals draw from a pool of distinct requests,
+    each arrival with its own output buffer; Poisson arrivals at 10..90% of
+    the closed-loop rate measured in the same run."""
     import ctypes
     rng = np.random.default_rng(0xC0DA6 + 5)
-    n_req = args.requests
-    sizes = np.exp(rng.uniform(np.log(64 << 10), np.log(256 << 20), n_req))
-    kinds = rng.choice(3, size=n_req, p=[0.4, 0.3, 0.3])
-    log(f"[bench] generating {n_req} requests")
-    reqs = []
-    for i in range(n_req):
+    n_pool, n_arr = args.pool, args.requests
+    sizes = np.exp(rng.uniform(np.log(64 << 10), np.log(256 << 20), n_pool))
+    kinds = rng.choice(3, size=n_pool, p=[0.4, 0.3, 0.3])
+    log(f"[bench] c5: generating a pool of {n_pool} requests")
+    pool = []
+    for i in range(n_pool):
         nch = max(1, int(round(sizes[i] / CH)))
         seed = 0xC0DA6 + 500 + i
         if kinds[i] == 0:
@@ -460,14 +498,26 @@ def bench_served(ctx, args):
             raw = ca.synth_deflate_payload(nch * CH, CH, seed=seed)
             comp, blk, _, n = ca.deflate_compress_chunks(raw, CH, level=6, threads=host_threads())
             off, ln, w, codec = blk[1:].copy(), np.zeros(1, np.uint64), 1, ca.DEFLATE
-        d = [ctx.upload(np.ascontiguousarray(comp)), ctx.upload(off), ctx.upload(ln), ctx.alloc(n * CH)]
+        d = [ctx.upload(np.ascontiguousarray(comp)), ctx.upload(off), ctx.upload(ln)]
+        pool.append(dict(d=d, codec=int(codec), w=w, n=n, comp_bytes=len(comp), raw=raw,
+                         bytes=len(raw)))
+    # the arrival sequence: pool entries drawn uniformly, each with its own output
+    pick = np.random.default_rng(0xC0DA6 + 6).integers(0, n_pool, n_arr)
+    out_off = np.concatenate([[0], np.cumsum([pool[p]["n"] * CH for p in pick])]).astype(np.int64)
+    log(f"[bench] c5: {n_arr} arrivals, {out_off[-1] >> 20} MiB of output buffers")
+    d_arena = ctx.alloc(int(out_off[-1]))
+    reqs = []
+    for a, p in enumerate(pick):
+        e = pool[p]
         q = ca.Request()
-        q.codec, q.flags, q.elem_bytes, q.chunk_bytes, q.n_chunks = codec, ca.REQ_DEVICE_IO, w, CH, n
-        q.comp, q.comp_bytes = d[0].ptr.value, len(comp)
-        q.blk_off, q.col_len, q.out, q.user_tag = d[1].ptr.value, d[2].ptr.value, d[3].ptr.value, i
-        reqs.append(dict(q=q, d=d, raw=raw, bytes=len(raw), codec=int(codec)))
-    mean_bytes = float(np.mean([r["bytes"] for r in reqs]))
-    total_bytes = sum(r["bytes"] for r in reqs)
+        q.codec, q.flags, q.elem_bytes, q.chunk_bytes, q.n_chunks = (e["codec"], ca.REQ_DEVICE_IO,
+                                                                    e["w"], CH, e["n"])
+        q.comp, q.comp_bytes = e["d"][0].ptr.value, e["comp_bytes"]
+        q.blk_off, q.col_len = e["d"][1].ptr.value, e["d"][2].ptr.value
+        q.out, q.user_tag = d_arena.offset(int(out_off[a])), a
+        reqs.append(q)
+    arr_bytes = np.array([pool[p]["bytes"] for p in pick], np.float64)
+    total_bytes = float(arr_bytes.sum())
 
     def run(inflight, arrivals=None):
         cfg = ca.ServerConfig(0, 200 * 10**9, inflight)
@@ -478,12 +528,12 @@ def bench_served(ctx, args):
         try:
             t0 = time.perf_counter()
             rid = ctypes.c_uint64()
-            for k, r in enumerate(reqs):
+            for k, q in enumerate(reqs):
                 if arrivals is not None:
                     dt = t0 + arrivals[k] - time.perf_counter()
                     if dt > 0:
                         time.sleep(dt)
-                ca.check(ca.lib().codag_server_submit(h, ctypes.byref(r["q"]), ctypes.byref(rid)),
+                ca.check(ca.lib().codag_server_submit(h, ctypes.byref(q), ctypes.byref(rid)),
                          "server_submit")
             while len(done) < len(reqs):
                 got = ctypes.c_uint32()
@@ -494,46 +544,73 @@ def bench_served(ctx, args):
         finally:
             ca.lib().codag_server_destroy(h)
         if any(c.status != 0 or c.failed_chunks for c in done):
-            raise SystemExit("served request failed")
+            raise SystemExit("c5: served request failed")
         lat = np.array([c.latency_seconds for c in done]) * 1e3
         return wall, lat
 
-    run(8)  # warm-up (device buffers, code objects)
-    for r in reqs:  # bit-exact check of every request's output
-        if r["d"][3].download(r["bytes"]).tobytes() != r["raw"].tobytes():
-            raise SystemExit(f"served output mismatch (request {r['q'].user_tag})")
-        r["raw"] = None
-    log("[bench] bit-exact check passed")
+    def check():
+        for a, p in enumerate(pick):
+            e = pool[p]
+            out = _slice(d_arena, int(out_off[a]), e["bytes"])
+            if not np.array_equal(out, e["raw"]):
+                raise SystemExit(f"c5: served output mismatch (arrival {a})")
+
+    run(8)  # warm-up (code objects, the server's memory pool)
+    check()
+    log("[bench] c5: bit-exact check of every arrival passed")
     wall1, _ = run(1)
     wall8, lat8 = run(8)
     peak = total_bytes / wall8
     levels = []
     for load in (0.1, 0.3, 0.5, 0.7, 0.9):
-        rate = load * peak / mean_bytes  # requests per second
-        arr = np.cumsum(np.random.default_rng(int(load * 100)).exponential(1.0 / rate, n_req))
+        rate = load * peak / (total_bytes / n_arr)  # requests per second
+        arr = np.cumsum(np.random.default_rng(int(load * 100)).exponential(1.0 / rate, n_arr))
         arr -= arr[0]
         wall, lat = run(8, arr)
         levels.append({"load": load, "offered_gbs": round(load * peak / 1e9, 2),
                        "achieved_gbs": round(total_bytes / wall / 1e9, 2),
                        "p50_ms": round(float(np.percentile(lat, 50)), 3),
                        "p99_ms": round(float(np.percentile(lat, 99)), 3)})
-        log(f"[bench] load {load}: {levels[-1]}")
+        log(f"[bench] c5: load {load}: {levels[-1]}")
+    check()
     return {
-        "metric": METRIC, "value": round(peak / 1e9, 2), "unit": "GB/s", "n_gpus": 1,
-        "steps": 1, "warmup": 1, "ms_per_step": round(wall8 * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "mixed",
-        "data": "synthetic", "bit_exact": True,
-        "config": {"workload": f"served mixed-codec decode, {n_req} requests (40% RLE v1 u64, "
-                               "30% RLE v2 u32, 30% deflate), log-uniform 64 KiB-256 MiB, "
-                               "device-resident, 200 GB HBM budget, 8 slots",
-                   "decompressed_bytes": int(total_bytes),
-                   "closed_loop_gbs": {"slots_8": round(peak / 1e9, 2),
-                                       "slots_1_sequential": round(total_bytes / wall1 / 1e9, 2)},
-                   "closed_loop_p50_ms": round(float(np.percentile(lat8, 50)), 3),
-                   "closed_loop_p99_ms": round(float(np.percentile(lat8, 99)), 3),
-                   "poisson": levels},
-        "roofline": None,
-        "cpu_baseline": None,
+        "workload": f"served mixed-codec decode, {n_arr} Poisson arrivals per load level drawn from "
+                    f"a pool of {n_pool} requests (40% RLE v1 u64, 30% RLE v2 u32, 30% deflate), "
+                    "log-uniform 64 KiB-256 MiB, device-resident, 200 GB HBM budget, 8 slots",
+        "value": round(peak / 1e9, 2), "unit": "GB/s", "ms_per_step": round(wall8 * 1e3, 3),
+        "dtype": "mixed", "bit_exact": True, "decompressed_bytes_per_level": int(total_bytes),
+        "closed_loop_gbs": {"slots_8": round(peak / 1e9, 2),
+                            "slots_1_sequential": round(total_bytes / wall1 / 1e9, 2)},
+        "closed_loop_p50_ms": round(float(np.percentile(lat8, 50)), 3),
+        "closed_loop_p99_ms": round(float(np.percentile(lat8, 99)), 3),
+        "poisson": levels, "roofline": None, "cpu_baseline": None,
+    }
+
+
+def _slice(buf, off, nbytes):
+    import ctypes
+    out = np.empty(nbytes, np.uint8)
+    if nbytes:
+        ca.check(ca.lib().codag_copy_d2h(buf.ctx.h, out.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_void_p(buf.offset(off)), nbytes), "copy_d2h")
+    return out
+
+
+# -------------------------------------------------------------------- main
+
+def headline(rec, args, extra_configs, device):
+    cfg = {k: v for k, v in rec.items()
+           if k not in ("value", "unit", "ms_per_step", "dtype", "roofline", "cpu_baseline",
+                        "bit_exact")}
+    cfg["device"] = device
+    if extra_configs:
+        cfg["configs"] = extra_configs
+    return {
+        "metric": METRIC, "value": rec["value"], "unit": rec["unit"], "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": rec["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": rec["dtype"],
+        "data": "synthetic", "bit_exact": True, "config": cfg,
+        "roofline": rec["roofline"], "cpu_baseline": rec["cpu_baseline"],
     }
 
 
@@ -542,22 +619,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="all", choices=["all", "c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--streams", type=int, default=10000, help="c4: number of zlib streams")
-    ap.add_argument("--requests", type=int, default=48, help="c5: requests per load level")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--pool", type=int, default=64, help="c5: distinct requests in the pool")
+    ap.add_argument("--requests", type=int, default=500, help="c5: arrivals per load level")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="c1/c2: keep the default launch shape instead of tuning per column")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="first run the plain copy kernel over MIB MiB (traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the decode shards "
-              "by chunk across GPUs without communication", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}). The decode shards "
+              "by chunk across GPUs without communication, so no multi-GPU figure is measured; "
+              "run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     ctx = ca.Context(0)
     name, cus = ctx.info()
-    log(f"[bench] device {name}, {cus} CUs")
+    device = f"{name} ({cus} CUs)"
+    log(f"[bench] device {device}; host: {host_threads()} cores usable, {cpu_model()}")
     if args.copy_ref:
         # one launch of the plain copy kernel moving a known byte count: under
         # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it scales the decode kernels'
@@ -570,16 +650,20 @@ def main():
         a.free()
         b.free()
         log(f"[bench] reference copy of {nb} bytes done")
-    if args.config in ("c1", "c2"):
-        res = bench_rle(ctx, args, args.config)
-    elif args.config == "c3":
-        res = bench_deflate(ctx, args)
-    elif args.config == "c4":
-        res = bench_batch(ctx, args)
-    else:
-        res = bench_served(ctx, args)
-    res["config"]["device"] = f"{name} ({cus} CUs)"
-    print(json.dumps(res), flush=True)
+    run = {"c1": lambda: bench_rle(ctx, args, "c1"), "c2": lambda: bench_rle(ctx, args, "c2"),
+           "c3": lambda: bench_deflate(ctx, args), "c4": lambda: bench_batch(ctx, args),
+           "c5": lambda: bench_served(ctx, args)}
+    if args.config != "all":
+        print(json.dumps(headline(run[args.config](), args, None, device)), flush=True)
+        return
+    t0 = time.perf_counter()
+    head = run["c4"]()
+    log(f"[bench] c4 done ({time.perf_counter() - t0:.0f} s)")
+    others = {}
+    for c in ("c1", "c2", "c3", "c5"):
+        others[c] = run[c]()
+        log(f"[bench] {c} done ({time.perf_counter() - t0:.0f} s)")
+    print(json.dumps(headline(head, args, others, device)), flush=True)
 
 
 if __name__ == "__main__":

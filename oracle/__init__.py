@@ -112,25 +112,33 @@ def rlev2_decode_chunk(stream, width: int, cap_values: int):
     return rc, out.view(_DT[width])[: produced.value]
 
 
-def rlev1_decode_chunks(comp, blk_off, col_len, width, chunk_bytes, threads=1):
+def _outbuf(out, nbytes):
+    """A caller's preallocated output (timing runs) or a fresh zeroed one."""
+    if out is None:
+        return np.zeros(nbytes, dtype=np.uint8)
+    assert out.dtype == np.uint8 and out.flags.c_contiguous and len(out) >= nbytes
+    return out
+
+
+def rlev1_decode_chunks(comp, blk_off, col_len, width, chunk_bytes, threads=1, out=None):
     """Whole RLE v1 container body; returns (status array, output bytes)."""
     comp = _bytes(comp)
     blk_off = np.ascontiguousarray(blk_off, dtype=np.uint64)
     col_len = np.ascontiguousarray(col_len, dtype=np.uint64)
     n = len(col_len)
-    out = np.zeros(n * chunk_bytes, dtype=np.uint8)
+    out = _outbuf(out, n * chunk_bytes)
     st = np.zeros(n, dtype=np.int32)
     lib().oracle_rlev1_decode_chunks(_p(comp), _p(blk_off, _u64p), _p(col_len, _u64p), n, width,
                                      chunk_bytes, _p(out), _p(st, _i32p), threads)
     return st, out
 
 
-def rlev2_decode_chunks(comp, blk_off, col_len, width, chunk_bytes, threads=1):
+def rlev2_decode_chunks(comp, blk_off, col_len, width, chunk_bytes, threads=1, out=None):
     comp = _bytes(comp)
     blk_off = np.ascontiguousarray(blk_off, dtype=np.uint64)
     col_len = np.ascontiguousarray(col_len, dtype=np.uint64)
     n = len(col_len)
-    out = np.empty(n * chunk_bytes, dtype=np.uint8)
+    out = _outbuf(out, n * chunk_bytes)
     st = np.zeros(n, dtype=np.int32)
     lib().oracle_rlev2_decode_chunks(_p(comp), _p(blk_off, _u64p), _p(col_len, _u64p), n, width,
                                      chunk_bytes, _p(out), _p(st, _i32p), threads)
@@ -163,26 +171,26 @@ def adler32(data) -> int:
     return int(lib().oracle_adler32(_p(pad), len(d)))
 
 
-def inflate_chunks(comp, offsets, chunk_bytes, threads=1):
+def inflate_chunks(comp, offsets, chunk_bytes, threads=1, out=None):
     """offsets: n+1 entries (stream i = comp[off[i]:off[i+1]])."""
     comp = _bytes(comp)
     off = np.ascontiguousarray(offsets, dtype=np.uint64)
     n = len(off) - 1
-    out = np.zeros(n * chunk_bytes, dtype=np.uint8)
+    out = _outbuf(out, n * chunk_bytes)
     st = np.zeros(n, dtype=np.int32)
     lib().oracle_inflate_chunks(_p(comp), _p(off, _u64p), n, chunk_bytes, _p(out),
                                 _p(st, _i32p), threads)
     return st, out
 
 
-def inflate_batch(src, src_off, src_len, dst_off, dst_cap, dst_total, threads=1):
+def inflate_batch(src, src_off, src_len, dst_off, dst_cap, dst_total, threads=1, out=None):
     src = _bytes(src)
     src_off = np.ascontiguousarray(src_off, dtype=np.uint64)
     src_len = np.ascontiguousarray(src_len, dtype=np.uint64)
     dst_off = np.ascontiguousarray(dst_off, dtype=np.uint64)
     dst_cap = np.ascontiguousarray(dst_cap, dtype=np.uint64)
     n = len(src_off)
-    dst = np.zeros(max(dst_total, 1), dtype=np.uint8)
+    dst = _outbuf(out, max(dst_total, 1))
     out_len = np.zeros(n, dtype=np.uint64)
     st = np.zeros(n, dtype=np.int32)
     lib().oracle_inflate_batch(_p(src), _p(src_off, _u64p), _p(src_len, _u64p), n, _p(dst),
