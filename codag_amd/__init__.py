@@ -122,6 +122,7 @@ def lib():
         _sig(L, "codag_rlev2_decode", st, dec)
         _sig(L, "codag_inflate_chunks", st, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _u32, _vp])
         _sig(L, "codag_inflate_batch", st, [_vp, _vp, _u64, _vp, _u32, _vp])
+        _sig(L, "codag_inflate_batch_noalloc", st, [_vp, _u64, _vp, _u32, _u32, _vp])
         dd = ctypes.POINTER(ctypes.c_double)
         _sig(L, "codag_rlev1_decode_host", st, [_vp, _vp, _u64, _vp, _u64, _vp, _u64, _u32, _u64,
                                                 _vp, _u64, p64, dd, dd])

@@ -386,6 +386,32 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
   return CODAG_OK;
 }
 
+codag_status codag_inflate_batch_noalloc(const codag_stream_desc *d_descs, uint64_t n,
+                                         codag_stream_result *d_results, uint32_t flags,
+                                         uint32_t groups_per_cu, codag_stream_t stream) {
+  if (n && (!d_descs || !d_results))
+    return fail(CODAG_ERR_INVALID, "inflate_batch_noalloc: null pointer");
+  if (n >= 0xffffffffull)
+    return fail(CODAG_ERR_INVALID,
+                "inflate_batch_noalloc: at most 2^32-2 streams per call (the work-queue "
+                "counter in the 256-byte static scratch array is 32 bits)");
+  if (groups_per_cu > 64)
+    return fail(CODAG_ERR_INVALID, "inflate_batch_noalloc: groups_per_cu must be 0..64");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  uint32_t *counter = nullptr;
+  HIP_TRY(inflate_scratch_ptr(&counter));
+  const uint64_t per_cu = groups_per_cu ? groups_per_cu : 28;
+  uint64_t groups = (uint64_t)(cus > 0 ? cus : 256) * per_cu;
+  if (n && groups > n) groups = n;
+  if (groups == 0) groups = 1;
+  HIP_TRY(launch_inflate_batch(d_descs, n, d_results, flags, counter, (uint32_t)groups,
+                               static_cast<hipStream_t>(stream)));
+  return CODAG_OK;
+}
+
 // ------------------------------------------------------------------ host wrappers
 
 static codag_status finish_host(codag_ctx *ctx, const DevBuf &d_status, uint64_t n,

@@ -36,6 +36,8 @@
 // (warp copy). Differences: stored blocks are decoded (the reference's branch
 // is empty, zlib.h:840-842), BTYPE 11 is rejected (zlib.h:843 treats it as
 // dynamic), the zlib header and Adler-32 trailer are checked as zlib does.
+#include <mutex>
+
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -1010,7 +1012,36 @@ inflate_batch_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
 
 __global__ void zero_counter_kernel(uint32_t *c) { *c = 0; }
 
+// Device memory of the allocation-free batch entry point: word 0 is its
+// work-queue counter. Every GPU holds its own copy for the life of the
+// process; all calls on one GPU share it.
+constexpr uint32_t kScratchWords = 64;  // 256 bytes
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+constexpr int kScratchDevices = 64;
+
 }  // namespace inflate
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (the host-side name is not a device
+// address and must never reach a kernel).
+hipError_t inflate_scratch_ptr(uint32_t **out) {
+  static std::mutex mu;
+  static uint32_t *ptrs[inflate::kScratchDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= inflate::kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(inflate::scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    ptrs[dev] = static_cast<uint32_t *>(p);
+  }
+  *out = ptrs[dev];
+  return hipSuccess;
+}
 
 hipError_t launch_inflate_chunks(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *off,
                                  uint64_t n,

@@ -40,6 +40,11 @@ hipError_t launch_inflate_batch(const codag_stream_desc *descs, uint64_t n,
                                 codag_stream_result *results, uint32_t flags, uint32_t *counter,
                                 uint32_t groups, hipStream_t stream);
 
+// Device address (on the current device) of the inflate kernels' static
+// 256-byte scratch array; its first word is the work-queue counter of the
+// allocation-free batch entry point.
+hipError_t inflate_scratch_ptr(uint32_t **out);
+
 // RLE v1, one walker wave and nexp (1 or 3) expander waves per chunk.
 hipError_t launch_rlev1_pc(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *blk_off,
                            const uint64_t *col_len, uint64_t n_chunks, uint32_t w,

@@ -139,6 +139,24 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
                                  codag_stream_result *d_results, uint32_t flags,
                                  codag_stream_t stream);
 
+/* Allocation-free variant of codag_inflate_batch. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It takes no context and runs on the current HIP device: the caller passes
+ * inputs and outputs, the work-queue counter lives in a static 256-byte
+ * device array of the library (one per GPU, held until the process ends), and
+ * the call makes no hipMalloc/hipFree and builds no graph of its own (a
+ * caller's stream capture records its two launches). The counter is reset by
+ * a kernel on `stream` (NULL = the device's null stream) before each launch.
+ * All calls on one GPU in a process share that counter and it keeps the last
+ * call's contents: calls that overlap on one GPU (different streams or host
+ * threads) race and must be serialized by the caller. Limits: n <= 2^32-2 streams per call
+ * (larger n is rejected with CODAG_ERR_INVALID); groups_per_cu 0..64 resident
+ * waves per CU (0 = 28). */
+codag_status codag_inflate_batch_noalloc(const codag_stream_desc *d_descs, uint64_t n,
+                                         codag_stream_result *d_results, uint32_t flags,
+                                         uint32_t groups_per_cu, codag_stream_t stream);
+
 /* ------------------------------------------------- host convenience (sync) */
 /* Used by the drop-in CLIs. Inputs are host buffers laid out exactly as the
  * reference's files; outputs are host buffers of at least the original
