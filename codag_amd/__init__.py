@@ -34,6 +34,8 @@ CHUNK_OK, CHUNK_TRUNCATED, CHUNK_OVERFLOW, CHUNK_CORRUPT = range(4)
 # inflate flags
 INFLATE_SKIP_ADLER = 1
 INFLATE_RAW = 2
+INFLATE_GZIP = 4
+INFLATE_CUDF_STATUS = 8
 # codecs
 RLEV1, RLEV2, DEFLATE = 1, 2, 3
 # server request flags

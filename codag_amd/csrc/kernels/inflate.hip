@@ -36,6 +36,7 @@
 // (warp copy). Differences: stored blocks are decoded (the reference's branch
 // is empty, zlib.h:840-842), BTYPE 11 is rejected (zlib.h:843 treats it as
 // dynamic), the zlib header and Adler-32 trailer are checked as zlib does.
+#include <cstddef>
 #include <mutex>
 
 #include "device.hpp"
@@ -122,6 +123,10 @@ struct Lds {
   alignas(4) uint8_t lens[320];
 };
 static_assert(sizeof(Lds::lens) >= 4 * (kWave + 16), "lens doubles as the token slots");
+// ring and lit are adjacent: their 4 KiB hold the CRC-32 slice tables while
+// no block is being decoded (gzip header and trailer)
+static_assert(offsetof(Lds, lit) == kRing && sizeof(Lds::ring) + sizeof(Lds::lit) == 4096,
+              "CRC tables need ring + lit");
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t j) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
@@ -333,6 +338,18 @@ __device__ __forceinline__ void gather_far(Out &o, uint32_t lane, uint64_t fm, u
   }
 }
 
+// floor(2^16 / d) + 1 for d = 1..63: (i * m[d]) >> 16 == i / d for every
+// i < 512. The product overshoots i / d by i * (m[d] - 2^16 / d) / 2^16 <
+// 512 / 2^16 = 1/128, while the fractional part of i / d is at most
+// 1 - 1/d <= 1 - 1/63, so the overshoot never carries into the next integer.
+struct Recip16 {
+  uint32_t m[64];
+  constexpr Recip16() : m() {
+    for (uint32_t d = 1; d < 64; d++) m[d] = 0x10000u / d + 1;
+  }
+};
+__constant__ Recip16 kRecip16 = Recip16();
+
 // Materialises a batch of nt tokens (lane j holds token j):
 //   literal: its LUT entry (bit 15 set, byte in bits 11:4);
 //   match: length in bits 8:0 (bit 15 clear), distance in bits 31:16.
@@ -398,16 +415,13 @@ __device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t to
       for (uint32_t i = kWave + lane; i < lj; i += kWave)
         o.ring[ridx(o, xj + i)] = o.ring[ridx(o, sj + i)];
     } else {
-      // periodic copy: byte i = source byte i mod dj. The float quotient is
-      // exact up to one unit for i < 2^9, dj < 64; the remainder is corrected.
-      const float rd = __builtin_amdgcn_rcpf((float)dj);
+      // periodic copy: byte i = source byte i mod dj, the quotient by an
+      // integer reciprocal multiply (exact for i < 512, dj < 64)
+      const uint32_t rm = kRecip16.m[dj];
 #pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
       for (uint32_t i = lane; i < lj; i += kWave) {
-        const uint32_t q = (uint32_t)((float)i * rd);
-        int32_t m = (int32_t)(i - q * dj);
-        if (m < 0) m += (int32_t)dj;
-        if (m >= (int32_t)dj) m -= (int32_t)dj;
-        o.ring[ridx(o, xj + i)] = o.ring[ridx(o, sj + (uint32_t)m)];
+        const uint32_t m = i - ((i * rm) >> 16) * dj;
+        o.ring[ridx(o, xj + i)] = o.ring[ridx(o, sj + m)];
       }
     }
   }
@@ -528,6 +542,15 @@ __device__ __forceinline__ int32_t slow_decode(const Huff<NSYM> &h, uint64_t buf
 }
 
 // ------------------------------------------------------------- stream decode
+
+// Status in the caller's convention: CODAG_CHUNK_* by default, cuDF's
+// gpu_inflate_status_s codes (cudf_deflate.cuh:1243-1343: 0 ok, 1 output
+// buffer too small, 2 input ended early, -1 invalid data) with
+// CODAG_INFLATE_CUDF_STATUS.
+__device__ __forceinline__ int32_t status_code(int32_t st, uint32_t flags) {
+  if (!(flags & CODAG_INFLATE_CUDF_STATUS) || st == kOk) return st;
+  return st == kOverflow ? 1 : (st == kTrunc ? 2 : -1);
+}
 
 struct Result {
   int32_t status;
@@ -835,6 +858,144 @@ __device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, u
   }
 }
 
+// ------------------------------------------------------------- CRC-32 (gzip)
+
+// CRC-32 of RFC 1952 (reflected polynomial 0xEDB88320). Polynomials mod P are
+// held bit-reflected: bit 31 is x^0. mulmod multiplies two of them (shift
+// and add over GF(2)); kX2n[k] is x^(2^k) mod P, so x^(8n) mod P is a product
+// of kX2n entries over the bits of n (the combination rule zlib's
+// crc32_combine uses: crc(A B) = crc(A) * x^(8|B|) + crc(B)).
+constexpr uint32_t kCrcPoly = 0xEDB88320u;
+
+__host__ __device__ constexpr uint32_t crc_mulmod(uint32_t a, uint32_t b) {
+  uint32_t m = 1u << 31, p = 0;
+  for (;;) {
+    if (a & m) {
+      p ^= b;
+      if ((a & (m - 1)) == 0) break;
+    }
+    m >>= 1;
+    b = (b & 1) ? (b >> 1) ^ kCrcPoly : b >> 1;
+  }
+  return p;
+}
+
+struct X2nTable {
+  uint32_t t[32];
+  constexpr X2nTable() : t() {
+    uint32_t p = 1u << 30;  // x^1
+    t[0] = p;
+    for (int k = 1; k < 32; k++) t[k] = p = crc_mulmod(p, p);
+  }
+};
+__constant__ X2nTable kX2n = X2nTable();
+
+// x^(8n) mod P.
+__device__ __forceinline__ uint32_t crc_x8n(uint32_t n) {
+  uint32_t p = 1u << 31;  // x^0
+  uint32_t k = 3;
+  while (n) {
+    if (n & 1) p = crc_mulmod(kX2n.t[k & 31], p);
+    n >>= 1;
+    k++;
+  }
+  return p;
+}
+
+// CRC-32 of n bytes at global address p, by the whole wave: four 256-entry
+// slice tables are built in `tab` (4 KiB of LDS), lane l takes the l-th of
+// 64 equal segments (slice-by-4), and the 64 segment CRCs are combined in
+// order. Every lane returns the same value.
+__device__ __forceinline__ uint32_t wave_crc32(const uint8_t *p, uint32_t n, uint32_t *tab) {
+  const uint32_t lane = lane_id();
+  // the bytes may have been stored by other lanes of this wave just before
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  for (uint32_t i = lane; i < 256; i += kWave) {
+    uint32_t c = i;
+#pragma unroll
+    for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ kCrcPoly : c >> 1;
+    tab[i] = c;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 1; t < 4; t++) {
+    for (uint32_t i = lane; i < 256; i += kWave) {
+      const uint32_t c = tab[(t - 1) * 256 + i];
+      tab[t * 256 + i] = (c >> 8) ^ tab[c & 0xff];
+    }
+    __syncthreads();
+  }
+  const uint32_t seg = ((n + kWave - 1) / kWave + 3) & ~3u;
+  const uint32_t lo = min(lane * seg, n), hi = min(lo + seg, n);
+  uint32_t c = 0xFFFFFFFFu;
+  uint32_t i = lo;
+  for (; i + 4 <= hi; i += 4) {
+    c ^= (uint32_t)p[i] | ((uint32_t)p[i + 1] << 8) | ((uint32_t)p[i + 2] << 16) |
+         ((uint32_t)p[i + 3] << 24);
+    c = tab[768 + (c & 0xff)] ^ tab[512 + ((c >> 8) & 0xff)] ^ tab[256 + ((c >> 16) & 0xff)] ^
+        tab[c >> 24];
+  }
+  for (; i < hi; i++) c = tab[(c ^ p[i]) & 0xff] ^ (c >> 8);
+  const uint32_t mine = ~c;
+  // x^(8 * segment length) of this lane's segment, every lane at once
+  const uint32_t shift = crc_x8n(hi - lo);
+  uint32_t total = 0;  // CRC of the empty string
+  for (uint32_t l = 0; l < (uint32_t)kWave; l++) {
+    const uint32_t cl = rdlane(mine, l), xl = rdlane(shift, l);
+    total = crc_mulmod(xl, total) ^ cl;
+  }
+  __syncthreads();
+  return uni(total);
+}
+
+// gzip member header (RFC 1952 2.3), checked as zlib's inflate() checks it:
+// ID1 ID2 = 1f 8b, CM = 8, no reserved FLG bits, FEXTRA / FNAME / FCOMMENT
+// skipped, FHCRC compared with the low 16 bits of the header's CRC-32.
+// Returns the header length, or -kTrunc / -kCorrupt.
+__device__ __forceinline__ int64_t gzip_header(const uint8_t *src, uint64_t len, uint32_t *tab) {
+  const uint32_t lane = lane_id();
+  if (len < 10) {
+    // zlib reports a short header as truncated unless a byte it has already
+    // read is wrong
+    const uint32_t b0 = len > 0 ? uni(src[0]) : 0x1f, b1 = len > 1 ? uni(src[1]) : 0x8b;
+    const uint32_t b2 = len > 2 ? uni(src[2]) : 8, b3 = len > 3 ? uni(src[3]) : 0;
+    if (b0 != 0x1f || b1 != 0x8b || b2 != 8 || (b3 & 0xe0)) return -kCorrupt;
+    return -kTrunc;
+  }
+  if (uni(src[0]) != 0x1f || uni(src[1]) != 0x8b || uni(src[2]) != 8) return -kCorrupt;
+  const uint32_t flg = uni(src[3]);
+  if (flg & 0xe0) return -kCorrupt;
+  uint64_t at = 10;
+  if (flg & 0x04) {  // FEXTRA
+    if (at + 2 > len) return -kTrunc;
+    const uint32_t xlen = uni(src[at]) | (uni(src[at + 1]) << 8);
+    at += 2 + xlen;
+    if (at > len) return -kTrunc;
+  }
+  for (uint32_t f = 0x08; f <= 0x10; f <<= 1) {  // FNAME, FCOMMENT: zero-terminated
+    if (!(flg & f)) continue;
+    for (;;) {
+      if (at >= len) return -kTrunc;
+      const uint64_t q = at + lane;
+      const bool z = q < len && src[q] == 0;
+      const uint64_t m = __ballot(z);
+      if (m) {
+        at += (uint64_t)__builtin_ctzll(m) + 1;
+        break;
+      }
+      at += kWave;
+    }
+  }
+  if (flg & 0x02) {  // FHCRC
+    if (at + 2 > len) return -kTrunc;
+    const uint32_t want = uni(src[at]) | (uni(src[at + 1]) << 8);
+    if ((wave_crc32(src, (uint32_t)at, tab) & 0xffff) != want) return -kCorrupt;
+    at += 2;
+  }
+  return (int64_t)at;
+}
+
 __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t len, uint8_t *dst, uint64_t cap,
                                  uint32_t flags, Lds &S) {
   const uint32_t lane = lane_id();
@@ -855,9 +1016,17 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
   b.w = reinterpret_cast<const uint32_t *>(a - lead);
   b.limit = (uint32_t)((lead + len) * 8);
   b.nw = (uint32_t)((lead + len + 3) / 4);
-  const bool raw = flags & CODAG_INFLATE_RAW;
+  const bool gzip = flags & CODAG_INFLATE_GZIP;
+  const bool raw = (flags & CODAG_INFLATE_RAW) && !gzip;
   uint64_t start = 0;
-  if (!raw) {
+  if (gzip) {
+    const int64_t h = gzip_header(src, len, reinterpret_cast<uint32_t *>(S.ring));
+    if (h < 0) {
+      res.status = (int32_t)(-h);
+      return res;
+    }
+    start = (uint64_t)h;
+  } else if (!raw) {
     if (len < 2) {
       res.status = kTrunc;
       return res;
@@ -944,7 +1113,26 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
   const uint32_t A = (uint32_t)((1 + s1) % 65521u);
   const uint32_t B = (uint32_t)((n * ((1 + s1) % 65521u) + 65521u * 65521u - s2) % 65521u);
   res.adler = uni((B << 16) | A);
-  if (st == kOk && !raw) {
+  if (gzip) {
+    // RFC 1952 trailer on the next byte boundary: CRC-32 and ISIZE, both
+    // little-endian; zlib checks the CRC first ("incorrect data check"),
+    // then the length ("incorrect length check")
+    res.adler = wave_crc32(dst, o.pos, reinterpret_cast<uint32_t *>(S.ring));
+    if (st == kOk) {
+      bits_drop(b, (uint32_t)((8 - (bits_pos(b) & 7)) & 7));
+      const uint64_t at = bits_pos(b) / 8 - lead;
+      if (at + 8 > len) {
+        st = kTrunc;
+      } else {
+        const uint32_t crc = uni(src[at]) | (uni(src[at + 1]) << 8) | (uni(src[at + 2]) << 16) |
+                             (uni(src[at + 3]) << 24);
+        const uint32_t isize = uni(src[at + 4]) | (uni(src[at + 5]) << 8) |
+                               (uni(src[at + 6]) << 16) | (uni(src[at + 7]) << 24);
+        if (!(flags & CODAG_INFLATE_SKIP_ADLER) && crc != res.adler) st = kCorrupt;
+        else if (isize != o.pos) st = kCorrupt;
+      }
+    }
+  } else if (st == kOk && !raw) {
     // RFC 1950 trailer: Adler-32, big-endian, on the next byte boundary
     bits_drop(b, (uint32_t)((8 - (bits_pos(b) & 7)) & 7));
     const uint64_t at = bits_pos(b) / 8 - lead;
@@ -975,7 +1163,7 @@ inflate_chunks_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
     r = inflate_stream(comp + o0, o1 - o0, out + c * chunk_bytes, chunk_bytes, flags, S);
   }
   if (lane_id() == 0) {
-    if (status) status[c] = r.status;
+    if (status) status[c] = status_code(r.status, flags);
     if (out_len) out_len[c] = r.bytes;
   }
 }
@@ -1003,7 +1191,7 @@ inflate_batch_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
     const Result r = inflate_stream(d.src, d.src_bytes, d.dst, d.dst_cap, flags, S);
     if (lane_id() == 0) {
       results[i].bytes_written = r.bytes;
-      results[i].status = r.status;
+      results[i].status = status_code(r.status, flags);
       results[i].adler32 = r.adler;
     }
     __syncthreads();

@@ -99,8 +99,19 @@ codag_status codag_rlev2_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t 
 
 /* ----------------------------------------------------------------- Deflate */
 enum {
-  CODAG_INFLATE_SKIP_ADLER = 1u << 0,   /* do not compare the Adler-32 trailer (still required) */
+  CODAG_INFLATE_SKIP_ADLER = 1u << 0,   /* do not compare the Adler-32 (gzip: CRC-32) trailer
+                                           (the trailer is still required) */
   CODAG_INFLATE_RAW = 1u << 1,          /* streams are raw deflate (no zlib header/trailer) */
+  /* streams are gzip members (RFC 1952), checked as zlib's inflate() checks
+   * them: ID 1f 8b, CM 8, no reserved FLG bits, FEXTRA/FNAME/FCOMMENT skipped,
+   * FHCRC verified, trailer CRC-32 and ISIZE verified. The result's adler32
+   * field then holds the CRC-32 of the bytes written. The header rules of
+   * cuDF's parse_gzip_header (cudf_deflate.cuh:1180-1227) are a subset. */
+  CODAG_INFLATE_GZIP = 1u << 2,
+  /* per-stream / per-chunk status codes in cuDF's convention
+   * (gpu_inflate_status_s, cudf_deflate.cuh:1243-1343): 0 ok, 1 output buffer
+   * too small, 2 input ended early, -1 invalid data, instead of CODAG_CHUNK_*. */
+  CODAG_INFLATE_CUDF_STATUS = 1u << 3,
 };
 
 /* Replaces deflate::decompress_gpu (include/zlib/zlib.h:1585-1700).
@@ -131,8 +142,8 @@ typedef struct codag_stream_desc {
 
 typedef struct codag_stream_result {
   uint64_t bytes_written;
-  int32_t status; /* CODAG_CHUNK_* */
-  uint32_t adler32;  /* Adler-32 of the bytes written */
+  int32_t status;    /* CODAG_CHUNK_* (cuDF codes with CODAG_INFLATE_CUDF_STATUS) */
+  uint32_t adler32;  /* Adler-32 of the bytes written (CRC-32 with CODAG_INFLATE_GZIP) */
 } codag_stream_result;
 
 codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_descs, uint64_t n,

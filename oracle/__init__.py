@@ -62,7 +62,7 @@ def lib():
         L.oracle_rlev2_closest_width.argtypes = [ctypes.c_uint32]
         L.oracle_rlev2_closest_width.restype = ctypes.c_uint32
         L.oracle_zlib_version.restype = ctypes.c_char_p
-        for fn in (L.oracle_zlib_inflate, L.oracle_raw_inflate):
+        for fn in (L.oracle_zlib_inflate, L.oracle_raw_inflate, L.oracle_gzip_inflate):
             fn.argtypes = [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64, _u64p]
         for fn in (L.oracle_crc32, L.oracle_adler32):
             fn.argtypes = [_u8p, ctypes.c_uint64]
@@ -149,12 +149,13 @@ def zlib_version() -> str:
     return lib().oracle_zlib_version().decode()
 
 
-def zlib_inflate(stream, out_cap: int, raw: bool = False):
+def zlib_inflate(stream, out_cap: int, raw: bool = False, gzip: bool = False):
     src = _bytes(stream)
     pad = np.concatenate([src, np.zeros(1, np.uint8)])
     out = np.zeros(max(out_cap, 1), dtype=np.uint8)
     got = ctypes.c_uint64(0)
-    fn = lib().oracle_raw_inflate if raw else lib().oracle_zlib_inflate
+    fn = lib().oracle_gzip_inflate if gzip else (
+        lib().oracle_raw_inflate if raw else lib().oracle_zlib_inflate)
     rc = fn(_p(pad), len(src), _p(out), out_cap, ctypes.byref(got))
     return rc, out[: got.value]
 

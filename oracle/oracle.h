@@ -75,6 +75,10 @@ const char *oracle_zlib_version(void);
 /* Full zlib stream (header + deflate + Adler-32 trailer checked by zlib). */
 int oracle_zlib_inflate(const uint8_t *in, uint64_t in_len, uint8_t *out, uint64_t out_cap,
                         uint64_t *out_len);
+/* gzip member (RFC 1952): header, deflate data, CRC-32 + ISIZE trailer,
+ * all checked by zlib (windowBits 31). */
+int oracle_gzip_inflate(const uint8_t *in, uint64_t in_len, uint8_t *out, uint64_t out_cap,
+                        uint64_t *out_len);
 /* Raw deflate data (no header/trailer). */
 int oracle_raw_inflate(const uint8_t *in, uint64_t in_len, uint8_t *out, uint64_t out_cap,
                        uint64_t *out_len);

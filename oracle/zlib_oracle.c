@@ -40,6 +40,11 @@ int oracle_zlib_inflate(const uint8_t *in, uint64_t in_len, uint8_t *out, uint64
   return do_inflate(in, in_len, out, out_cap, out_len, 15);
 }
 
+int oracle_gzip_inflate(const uint8_t *in, uint64_t in_len, uint8_t *out, uint64_t out_cap,
+                        uint64_t *out_len) {
+  return do_inflate(in, in_len, out, out_cap, out_len, 31);
+}
+
 int oracle_raw_inflate(const uint8_t *in, uint64_t in_len, uint8_t *out, uint64_t out_cap,
                        uint64_t *out_len) {
   return do_inflate(in, in_len, out, out_cap, out_len, -15);

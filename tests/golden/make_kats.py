@@ -61,7 +61,8 @@ def v2_vectors():
     # DIRECT: width code 15 -> 16 bits, length 4, big-endian 16-bit values
     vs.append(dict(name="v2_direct", hex="5e035ca1ab1edeadbeef", width=4,
                    values=[0x5CA1, 0xAB1E, 0xDEAD, 0xBEEF]))
-    # DELTA: width code 2 (2 bits), length 10, base 2, delta base 1, deltas 0,1,1,0,1,0,1,0 ...
+    # DELTA: width code 3 (4 bits), length 10, base 2, delta base 1, then the
+    # 4-bit deltas 2,2,4,2,4,2,4,6
     primes = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29]
     vs.append(dict(name="v2_delta", hex="c609020222424246", width=8, values=primes))
     # PATCHED_BASE: base 2000 (2 bytes), width 5 bits, 20 values, one patch

@@ -369,3 +369,78 @@ def test_batch_errors_and_overflow(ctx):
     status, written, adler, dst, doff = _batch(ctx, streams, caps)
     exp = [oracle.zlib_inflate(s, c)[0] for s, c in zip(streams, caps)]
     assert status.tolist() == exp
+
+
+def gz(data, level=6, name=None, comment=None, extra=None, hcrc=False, mtime=1234567):
+    """A gzip member (RFC 1952) with the optional header fields."""
+    flg = (0x08 if name is not None else 0) | (0x10 if comment is not None else 0) | \
+        (0x04 if extra is not None else 0) | (0x02 if hcrc else 0)
+    h = bytes([0x1F, 0x8B, 8, flg]) + mtime.to_bytes(4, "little") + bytes([0, 3])
+    if extra is not None:
+        h += len(extra).to_bytes(2, "little") + extra
+    if name is not None:
+        h += name + b"\0"
+    if comment is not None:
+        h += comment + b"\0"
+    if hcrc:
+        h += (zlib.crc32(h) & 0xFFFF).to_bytes(2, "little")
+    body = z(data, level, wbits=-15)
+    return h + body + zlib.crc32(data).to_bytes(4, "little") + \
+        (len(data) & 0xFFFFFFFF).to_bytes(4, "little")
+
+
+def _gz_cases():
+    data = ca.synth_deflate_payload(300000, CH, seed=77).tobytes()
+    good = gz(data, name=b"col.bin", comment=b"x" * 150, extra=b"ab" * 40, hcrc=True)
+    bad_hcrc = bytearray(good)
+    bad_hcrc[10 + 2 + 80 + 8 + 151] ^= 1  # the FHCRC bytes follow the comment's terminator
+    streams = [gz(data), gz(data[:1000], 9), gz(b""), good, bytes(bad_hcrc),
+               good[:5], good[:-3],                       # truncated header / trailer
+               good[:3] + bytes([good[3] | 0x20]) + good[4:],  # reserved flag bit
+               good[:-8] + (zlib.crc32(data) ^ 1).to_bytes(4, "little") + good[-4:],  # CRC
+               good[:-4] + (len(data) + 1).to_bytes(4, "little"),                     # ISIZE
+               bytes([0x1F, 0x8C]) + good[2:]]             # magic
+    # the stream cut inside its trailer gets one spare output byte: with the
+    # output exactly full, zlib cannot tell a short input from a full buffer
+    caps = [len(data), 1000, 16, len(data), len(data), 16, len(data) + 1, len(data), len(data),
+            len(data), len(data)]
+    return data, streams, caps
+
+
+def test_gzip_members_match_zlib(ctx):
+    say("gzip members against zlib (windowBits 31)")
+    data, streams, caps = _gz_cases()
+    status, written, crc, dst, doff = _batch(ctx, streams, caps, ca.INFLATE_GZIP)
+    for i, (s, c) in enumerate(zip(streams, caps)):
+        rc, ref = oracle.zlib_inflate(s, c, gzip=True)
+        assert status[i] == rc, (i, status[i], rc)
+        if rc == 0:
+            assert int(written[i]) == len(ref)
+            assert dst[int(doff[i]): int(doff[i]) + len(ref)].tobytes() == ref.tobytes()
+            assert int(crc[i]) == zlib.crc32(ref.tobytes())
+    assert status.tolist()[:4] == [0, 0, 0, 0]
+    # the CRC check can be skipped; ISIZE is still checked
+    status, written, crc, dst, doff = _batch(ctx, streams[8:10], caps[8:10],
+                                             ca.INFLATE_GZIP | ca.INFLATE_SKIP_ADLER)
+    assert status.tolist() == [0, ca.CHUNK_CORRUPT]
+
+
+def test_gzip_chunks_entry_point(ctx):
+    say("gzip through codag_inflate_chunks")
+    raw = ca.synth_deflate_payload(3 * CH, CH, seed=78).tobytes()
+    chunks = [gz(raw[i * CH:(i + 1) * CH], name=b"c%d" % i) for i in range(3)]
+    comp, off = container(chunks)
+    st, ln, out = inflate(ctx, comp, off, CH, ca.INFLATE_GZIP)
+    assert st.tolist() == [0, 0, 0] and out[: 3 * CH].tobytes() == raw
+
+
+def test_cudf_status_codes(ctx):
+    say("cuDF status convention")
+    data = ca.synth_deflate_payload(50000, CH, seed=79).tobytes()
+    good = z(data, 6)
+    streams = [good, good, good[:100], b"\x78\x9c\xff\xff\xff"]
+    caps = [len(data), 1000, len(data), 100]
+    plain, *_ = _batch(ctx, streams, caps)
+    assert plain.tolist() == [ca.CHUNK_OK, ca.CHUNK_OVERFLOW, ca.CHUNK_TRUNCATED, ca.CHUNK_CORRUPT]
+    cudf, *_ = _batch(ctx, streams, caps, ca.INFLATE_CUDF_STATUS)
+    assert cudf.tolist() == [0, 1, 2, -1]  # cudf_deflate.cuh:1243-1343

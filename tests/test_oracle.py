@@ -280,3 +280,11 @@ def test_v2_overlong_varint_is_corrupt():
     # truncated inside the varint: TRUNCATED wins
     rc, _ = oracle.rlev2_decode_chunk(bytes([0xC0, 0x02]) + bytes([0x80] * 12), 8, 64)
     assert rc == oracle.TRUNCATED
+
+
+def test_periodic_copy_reciprocal_is_exact():
+    # the inflate kernel's periodic match copy takes i // d as
+    # (i * (2^16 // d + 1)) >> 16 for i < 512 (match bytes), d < 64 (distance)
+    for d in range(1, 64):
+        m = 0x10000 // d + 1
+        assert all(((i * m) >> 16) == i // d for i in range(512))
