@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X CODAG decoders (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--config all|c1|c2|c3|c4|c5|brle|lzss]
 
 Prints ONE JSON line. With the default ``--config all`` the line is headlined
 on the largest single-GPU workload -- the batched uneven Deflate decode of
@@ -13,6 +14,8 @@ under ``config.configs``:
 * c3  Deflate decode of 1 GiB in 8192 zlib level-6 chunks of 128 KiB
 * c4  batched uneven Deflate (the headline)
 * c5  served mixed-codec requests, Poisson arrivals, p50/p99 latency
+* brle  byte RLE decode of 1 GiB in 4096 chunks of 256 KiB (brle.h's format)
+* lzss  LZSS decode of 1 GiB in 8192 chunks of 128 KiB (lzss.h's format)
 
 A step is one pass of the hot path over the whole workload with the
 compressed input already resident in HBM. Every workload's GPU output is
@@ -347,6 +350,94 @@ def bench_deflate(ctx, args):
     }
 
 
+# ------------------------------------------------------ BRLE / LZSS
+
+def bench_bytecodec(ctx, args, kind):
+    """BRLE (kind "brle") or LZSS ("lzss") over 1 GiB of its benchmark bytes,
+    encoded by the reference encoders' restatements."""
+    lz = kind == "lzss"
+    chunk = ca.LZSS_CHUNK_BYTES if lz else ca.BRLE_CHUNK_BYTES
+    n_bytes = 1 << 30
+    log(f"[bench] {kind}: generating and encoding 1 GiB")
+    raw = ca.synth_byte_payload(n_bytes, 1 if lz else 0, seed=0xC0DA6 + (11 if lz else 10))
+    if lz:
+        data, idx, chunk, _ = ca.lzss_container(ca.lzss_encode(raw, chunk, host_threads()))
+        n = len(idx)
+    else:
+        data, idx, _ = ca.brle_container(ca.brle_encode(raw, chunk, host_threads()))
+        n = len(idx) - 1
+    data = np.ascontiguousarray(data)
+    idx = np.ascontiguousarray(idx, dtype=np.uint64)
+    d_comp = ctx.upload(data)
+    d_idx = ctx.upload(idx)
+    d_out = ctx.alloc(n * chunk)
+    d_len = ctx.alloc(8 * n)
+    d_st = ctx.alloc(4 * n)
+    fn = ctx.lzss_decode if lz else ctx.brle_decode
+
+    def launch():
+        fn(d_comp, len(data), d_idx, n, chunk, d_out, d_len, d_st)
+
+    def check():
+        st = d_st.download(4 * n, np.int32)
+        out = d_out.download(n_bytes)
+        if not (np.all(st == 0) and np.array_equal(out, raw)):
+            raise SystemExit(f"{kind}: bit-exact check failed")
+        return crc32(out)
+
+    launch()
+    ctx.sync()
+    crc = check()
+    log(f"[bench] {kind}: bit-exact check passed")
+    for _ in range(args.warmup):
+        launch()
+    ctx.sync()
+    ctx.event(0)
+    for _ in range(args.steps):
+        launch()
+    ctx.event(1)
+    ms = ctx.elapsed(0, 1) / args.steps
+    if check() != crc:
+        raise SystemExit(f"{kind}: output changed during the timed steps")
+    alg = len(data) + n_bytes + 8 * len(idx)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        dec = oracle.lzss_decode_chunks if lz else oracle.brle_decode_chunks
+        out_h = np.empty(n * chunk, np.uint8)
+        parts = np.array_split(np.arange(n), host_threads())
+
+        def part(ids):
+            if len(ids) == 0:
+                return
+            a, b = int(ids[0]), int(ids[-1]) + 1
+            if lz:
+                lo = int(idx[a - 1]) if a else 0
+                sub = idx[a:b] - np.uint64(lo)
+                dec(data[lo:int(idx[b - 1])], sub, chunk, out=out_h[a * chunk:b * chunk])
+            else:
+                lo = int(idx[a])
+                sub = idx[a:b + 1] - np.uint64(lo)
+                dec(data[lo:int(idx[b])], sub, chunk, out=out_h[a * chunk:b * chunk])
+
+        k1 = 256 if lz else 128
+        cpu = cpu_leg(lambda: _parallel(part, parts), n_bytes,
+                      f"the whole workload ({n} chunks, {n_bytes >> 20} MiB out)",
+                      lambda: part(np.arange(k1)), k1 * chunk,
+                      f"the first {k1} chunks ({k1 * chunk >> 20} MiB out)",
+                      f"oracle C {kind} decoder")
+    what = ("LZSS (2 KiB history, matches 3-34) of word text" if lz else
+            "byte RLE of runs (3 + Geometric(mean 30)) and literal stretches (1-64)")
+    return {
+        "workload": f"{kind} 1 GiB, {n} x {chunk >> 10} KiB chunks, {what}",
+        "value": round(n_bytes / (ms / 1e3) / 1e9, 2), "unit": "GB/s", "ms_per_step": round(ms, 4),
+        "dtype": "u8", "bit_exact": True, "chunks": int(n), "compressed_bytes": int(len(data)),
+        "decompressed_bytes": n_bytes, "crc32": crc,
+        "roofline": roofline(f"{kind}_chunks_kernel", ms, alg, f"{kind}_chunks_kernel:{kind}:{n}"),
+        "cpu_baseline": cpu,
+    }
+
+
 # ----------------------------------------------------------- batched (c4)
 
 def _parallel(fn, items):
@@ -619,7 +710,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="all", choices=["all", "c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="all",
+                    choices=["all", "c1", "c2", "c3", "c4", "c5", "brle", "lzss"])
     ap.add_argument("--streams", type=int, default=10000, help="c4: number of zlib streams")
     ap.add_argument("--pool", type=int, default=64, help="c5: distinct requests in the pool")
     ap.add_argument("--requests", type=int, default=500, help="c5: arrivals per load level")
@@ -652,7 +744,9 @@ def main():
         log(f"[bench] reference copy of {nb} bytes done")
     run = {"c1": lambda: bench_rle(ctx, args, "c1"), "c2": lambda: bench_rle(ctx, args, "c2"),
            "c3": lambda: bench_deflate(ctx, args), "c4": lambda: bench_batch(ctx, args),
-           "c5": lambda: bench_served(ctx, args)}
+           "c5": lambda: bench_served(ctx, args),
+           "brle": lambda: bench_bytecodec(ctx, args, "brle"),
+           "lzss": lambda: bench_bytecodec(ctx, args, "lzss")}
     if args.config != "all":
         print(json.dumps(headline(run[args.config](), args, None, device)), flush=True)
         return
@@ -660,7 +754,7 @@ def main():
     head = run["c4"]()
     log(f"[bench] c4 done ({time.perf_counter() - t0:.0f} s)")
     others = {}
-    for c in ("c1", "c2", "c3", "c5"):
+    for c in ("c1", "c2", "c3", "c5", "brle", "lzss"):
         others[c] = run[c]()
         log(f"[bench] {c} done ({time.perf_counter() - t0:.0f} s)")
     print(json.dumps(headline(head, args, others, device)), flush=True)

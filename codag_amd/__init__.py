@@ -1,4 +1,4 @@
-"""codag_amd -- MI355X-native CODAG decompressors (RLE v1, RLE v2, Deflate).
+"""codag_amd -- MI355X-native CODAG decompressors (RLE v1, RLE v2, Deflate, BRLE, LZSS).
 
 Python host mirror of the C ABI in ``include/codag.h`` (``lib/libcodag.so``),
 loaded with ctypes. The GPU work happens in the library's gfx950 HIP kernels;
@@ -13,6 +13,9 @@ The sub-namespaces mirror the reference's host interfaces:
   (include/rlev2/encoder_trans.h:482-564, decoder_trans.h:57-133)
 * :class:`deflate` -- ``deflate::decompress_gpu`` (include/zlib/zlib.h:1583-1700)
   and the container writer of deflate_compressor.py:44-126
+* :class:`brle`    -- ``brle::compress_gpu`` / ``decompress_gpu`` (include/brle/brle.h:251-342)
+* :class:`lzss`    -- ``lzss::compress_gpu`` / ``decompress_gpu_base``
+  (include/lzss/lzss.h:1060-1189)
 """
 from __future__ import annotations
 
@@ -42,6 +45,8 @@ RLEV1, RLEV2, DEFLATE = 1, 2, 3
 REQ_DEVICE_IO = 1
 
 CHUNK_BYTES = 131072
+BRLE_CHUNK_BYTES = 262144  # include/brle/brle.h:8
+LZSS_CHUNK_BYTES = 131072  # include/lzss/lzss.h:21
 
 _LIB = None
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -125,6 +130,8 @@ def lib():
         _sig(L, "codag_inflate_chunks", st, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _u32, _vp])
         _sig(L, "codag_inflate_batch", st, [_vp, _vp, _u64, _vp, _u32, _vp])
         _sig(L, "codag_inflate_batch_noalloc", st, [_vp, _u64, _vp, _u32, _u32, _vp])
+        _sig(L, "codag_brle_decode", st, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp])
+        _sig(L, "codag_lzss_decode", st, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp])
         dd = ctypes.POINTER(ctypes.c_double)
         _sig(L, "codag_rlev1_decode_host", st, [_vp, _vp, _u64, _vp, _u64, _vp, _u64, _u32, _u64,
                                                 _vp, _u64, p64, dd, dd])
@@ -132,7 +139,12 @@ def lib():
                                                 _u64, p64, dd, dd])
         _sig(L, "codag_inflate_chunks_host", st, [_vp, _vp, _u64, _vp, _u64, _u64, _u32, _vp, _u64,
                                                   p64, dd, dd])
+        _sig(L, "codag_brle_decode_host", st, [_vp, _vp, _u64, _u64, _vp, _u64, p64, dd, dd])
+        _sig(L, "codag_lzss_decode_host", st, [_vp, _vp, _u64, _vp, _u64, p64, dd, dd])
         _sig(L, "codag_host_free", None, [_vp])
+        _sig(L, "codag_brle_encode", st, [_vp, _u64, _u64, ctypes.c_int, pp, p64])
+        _sig(L, "codag_lzss_encode", st, [_vp, _u64, _u64, ctypes.c_int, pp, p64])
+        _sig(L, "codag_synth_byte_payload", st, [_u64, _u32, _u64, pp])
         _sig(L, "codag_rlev1_encode", st, [_vp, _u64, _u32, _u64, pp, p64, pp, p64, pp, p64, p64])
         _sig(L, "codag_rlev2_encode", st, [_vp, _u64, _u32, _u64, pp, p64, pp, pp, p64])
         _sig(L, "codag_deflate_compress_chunks", st, [_vp, _u64, _u64, ctypes.c_int, ctypes.c_int,
@@ -299,6 +311,16 @@ class Context:
         check(lib().codag_inflate_batch(self.h, _dp(descs), n, _dp(results), flags, stream),
               "inflate_batch")
 
+    def brle_decode(self, comp, comp_bytes, ptr, n, chunk, out, out_len=None, status=None,
+                    stream=None):
+        check(lib().codag_brle_decode(self.h, _dp(comp), comp_bytes, _dp(ptr), n, chunk, _dp(out),
+                                      _dp(out_len), _dp(status), stream), "brle_decode")
+
+    def lzss_decode(self, comp, comp_bytes, ends, n, chunk, out, out_len=None, status=None,
+                    stream=None):
+        check(lib().codag_lzss_decode(self.h, _dp(comp), comp_bytes, _dp(ends), n, chunk, _dp(out),
+                                      _dp(out_len), _dp(status), stream), "lzss_decode")
+
     def tune_get(self, codec) -> TuneParams:
         p = TuneParams()
         check(lib().codag_tune_get(self.h, codec, ctypes.byref(p)), "tune_get")
@@ -437,6 +459,49 @@ def synth_rlev2(n_values, elem_bytes=8, chunk_bytes=CHUNK_BYTES, seed=0xC0DA8):
         _take(l, nc * 8, np.uint64), nc
 
 
+def brle_encode(raw, chunk_bytes=BRLE_CHUNK_BYTES, threads=0) -> np.ndarray:
+    """brle::compress_gpu's container (u8): [u32 n+1][u64 ptr[n+1]][u64 len][streams]."""
+    a = _as_u8(raw)
+    c, cn = _c(), _u64()
+    check(lib().codag_brle_encode(_ptr(a) if len(a) else None, len(a), chunk_bytes, threads,
+                                  ctypes.byref(c), ctypes.byref(cn)), "brle_encode")
+    return _take(c, cn.value)
+
+
+def lzss_encode(raw, chunk_bytes=LZSS_CHUNK_BYTES, threads=0) -> np.ndarray:
+    """lzss::compress_gpu's container (u8): [u64 len][u32 x5][u64 ends[n]][streams]."""
+    a = _as_u8(raw)
+    c, cn = _c(), _u64()
+    check(lib().codag_lzss_encode(_ptr(a) if len(a) else None, len(a), chunk_bytes, threads,
+                                  ctypes.byref(c), ctypes.byref(cn)), "lzss_encode")
+    return _take(c, cn.value)
+
+
+def brle_container(container):
+    """Splits a BRLE container into (data u8, ptr u64 [n+1], original length)."""
+    c = _as_u8(container)
+    n_ptr = int(c[:4].view(np.uint32)[0])
+    ptr = c[4:4 + 8 * n_ptr].view(np.uint64)
+    orig = int(c[4 + 8 * n_ptr:12 + 8 * n_ptr].view(np.uint64)[0])
+    return c[12 + 8 * n_ptr:], ptr, orig
+
+
+def lzss_container(container):
+    """Splits an LZSS container into (data u8, ends u64 [n], chunk bytes, original length)."""
+    c = _as_u8(container)
+    orig = int(c[:8].view(np.uint64)[0])
+    chunk, n = (int(x) for x in c[8:16].view(np.uint32))
+    ends = c[28:28 + 8 * n].view(np.uint64)
+    return c[28 + 8 * n:], ends, chunk, orig
+
+
+def synth_byte_payload(n_bytes, kind, seed=0xC0DAB):
+    """kind 0: BRLE benchmark column (runs and literal stretches); 1: LZSS text."""
+    r = _c()
+    check(lib().codag_synth_byte_payload(n_bytes, kind, seed, ctypes.byref(r)), "synth_byte_payload")
+    return _take(r, n_bytes)
+
+
 def synth_deflate_payload(n_bytes, part_bytes=CHUNK_BYTES, seed=0xC0DA9):
     r = _c()
     check(lib().codag_synth_deflate_payload(n_bytes, part_bytes, seed, ctypes.byref(r)),
@@ -513,4 +578,43 @@ class deflate:
                                               _ptr(out), len(out), ctypes.byref(on),
                                               ctypes.byref(ks), ctypes.byref(ts)),
               "inflate_chunks_host")
+        return out[: on.value], ks.value
+
+
+class brle:
+    """Mirror of namespace brle (include/brle/brle.h)."""
+
+    @staticmethod
+    def compress_gpu(raw, chunk_size=BRLE_CHUNK_BYTES, threads=0):
+        return brle_encode(raw, chunk_size, threads)
+
+    @staticmethod
+    def decompress_gpu(ctx: Context, container, chunk_size=BRLE_CHUNK_BYTES):
+        """Returns (decoded bytes, kernel seconds). Raises CodagError on failure."""
+        c = _as_u8(container)
+        _, _, orig = brle_container(c)
+        out = np.empty(max(orig, 1), dtype=np.uint8)
+        on, ks, ts = _u64(), ctypes.c_double(), ctypes.c_double()
+        check(lib().codag_brle_decode_host(ctx.h, _ptr(c), len(c), chunk_size, _ptr(out), len(out),
+                                           ctypes.byref(on), ctypes.byref(ks), ctypes.byref(ts)),
+              "brle_decode_host")
+        return out[: on.value], ks.value
+
+
+class lzss:
+    """Mirror of namespace lzss (include/lzss/lzss.h)."""
+
+    @staticmethod
+    def compress_gpu(raw, chunk_size=LZSS_CHUNK_BYTES, threads=0):
+        return lzss_encode(raw, chunk_size, threads)
+
+    @staticmethod
+    def decompress_gpu_base(ctx: Context, container):
+        c = _as_u8(container)
+        orig = int(c[:8].view(np.uint64)[0]) if len(c) >= 8 else 0
+        out = np.empty(max(orig, 1), dtype=np.uint8)
+        on, ks, ts = _u64(), ctypes.c_double(), ctypes.c_double()
+        check(lib().codag_lzss_decode_host(ctx.h, _ptr(c), len(c), _ptr(out), len(out),
+                                           ctypes.byref(on), ctypes.byref(ks), ctypes.byref(ts)),
+              "lzss_decode_host")
         return out[: on.value], ks.value

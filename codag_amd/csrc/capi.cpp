@@ -292,6 +292,42 @@ codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_
   return CODAG_OK;
 }
 
+static codag_status byte_codec_args(const char *what, codag_ctx *ctx, const uint8_t *d_comp,
+                                     const uint64_t *d_idx, uint64_t n_chunks,
+                                     uint64_t chunk_bytes, const uint8_t *d_out) {
+  if (!ctx) return fail(CODAG_ERR_INVALID, std::string(what) + ": null ctx");
+  if (n_chunks && (!d_comp || !d_idx || !d_out))
+    return fail(CODAG_ERR_INVALID, std::string(what) + ": null device pointer");
+  if (chunk_bytes == 0 || chunk_bytes >= (1ull << 31))
+    return fail(CODAG_ERR_INVALID, std::string(what) + ": chunk_bytes must be in [1, 2^31)");
+  if (n_chunks > 0x7fffffffull) return fail(CODAG_ERR_INVALID, std::string(what) + ": too many chunks");
+  return CODAG_OK;
+}
+
+codag_status codag_brle_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
+                               const uint64_t *d_ptr, uint64_t n_chunks, uint64_t chunk_bytes,
+                               uint8_t *d_out, uint64_t *d_out_len, int32_t *d_status,
+                               codag_stream_t stream) {
+  if (codag_status s = byte_codec_args("brle_decode", ctx, d_comp, d_ptr, n_chunks, chunk_bytes, d_out))
+    return s;
+  DeviceGuard g(ctx->device);
+  HIP_TRY(launch_brle_chunks(d_comp, comp_bytes, d_ptr, n_chunks, chunk_bytes, d_out, d_out_len,
+                             d_status, pick(ctx, stream)));
+  return CODAG_OK;
+}
+
+codag_status codag_lzss_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
+                               const uint64_t *d_ends, uint64_t n_chunks, uint64_t chunk_bytes,
+                               uint8_t *d_out, uint64_t *d_out_len, int32_t *d_status,
+                               codag_stream_t stream) {
+  if (codag_status s = byte_codec_args("lzss_decode", ctx, d_comp, d_ends, n_chunks, chunk_bytes, d_out))
+    return s;
+  DeviceGuard g(ctx->device);
+  HIP_TRY(launch_lzss_chunks(d_comp, comp_bytes, d_ends, n_chunks, chunk_bytes, d_out, d_out_len,
+                             d_status, pick(ctx, stream)));
+  return CODAG_OK;
+}
+
 // The counter slot of stream s: the slot s already owns, else a free one,
 // else the least recently used, after waiting for its last launch (and
 // dropping the cached graph, which refers to the slot's counter).
@@ -611,6 +647,118 @@ codag_status codag_inflate_chunks_host(codag_ctx *ctx, const uint8_t *comp, uint
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   *out_bytes = orig;
   s = finish_host(ctx, d_st, n, kernel_seconds);
+  if (total_seconds) *total_seconds = now_s() - t0;
+  return s;
+}
+
+// Shared tail of the BRLE / LZSS host wrappers: uploads the streams and the
+// index array, decodes n chunks of chunk_bytes, checks that chunk i produced
+// exactly min(chunk_bytes, orig - i*chunk_bytes) bytes and copies orig bytes
+// out.
+static codag_status byte_codec_host(codag_ctx *ctx, bool lzss, const uint8_t *data,
+                                    uint64_t data_bytes, const uint64_t *idx, uint64_t idx_n,
+                                    uint64_t n, uint64_t chunk_bytes, uint64_t orig, uint8_t *out,
+                                    uint64_t *out_bytes, double *kernel_seconds) {
+  DeviceGuard g(ctx->device);
+  DevBuf d_in, d_idx, d_out, d_len, d_st;
+  HIP_TRY(d_in.alloc(data_bytes));
+  HIP_TRY(d_idx.alloc(idx_n * 8));
+  HIP_TRY(d_out.alloc(n * chunk_bytes));
+  HIP_TRY(d_len.alloc(n * 8));
+  HIP_TRY(d_st.alloc(n * 4));
+  if (data_bytes) HIP_TRY(hipMemcpyAsync(d_in.p, data, data_bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (idx_n) HIP_TRY(hipMemcpyAsync(d_idx.p, idx, idx_n * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  codag_status s =
+      lzss ? codag_lzss_decode(ctx, d_in.as<uint8_t>(), data_bytes, d_idx.as<uint64_t>(), n,
+                               chunk_bytes, d_out.as<uint8_t>(), d_len.as<uint64_t>(),
+                               d_st.as<int32_t>(), ctx->stream)
+           : codag_brle_decode(ctx, d_in.as<uint8_t>(), data_bytes, d_idx.as<uint64_t>(), n,
+                               chunk_bytes, d_out.as<uint8_t>(), d_len.as<uint64_t>(),
+                               d_st.as<int32_t>(), ctx->stream);
+  if (s) return s;
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  if (orig) HIP_TRY(hipMemcpyAsync(out, d_out.p, orig, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *out_bytes = orig;
+  s = finish_host(ctx, d_st, n, kernel_seconds);
+  if (s) return s;
+  std::vector<uint64_t> got(n);
+  if (n) HIP_TRY(hipMemcpy(got.data(), d_len.p, n * 8, hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < n; i++) {
+    const uint64_t want = std::min(chunk_bytes, orig - i * chunk_bytes);
+    if (got[i] != want)
+      return fail(CODAG_ERR_DECODE, "chunk " + std::to_string(i) + " produced " +
+                                        std::to_string(got[i]) + " bytes, expected " +
+                                        std::to_string(want));
+  }
+  return CODAG_OK;
+}
+
+codag_status codag_brle_decode_host(codag_ctx *ctx, const uint8_t *c, uint64_t cb,
+                                    uint64_t chunk_bytes, uint8_t *out, uint64_t out_cap,
+                                    uint64_t *out_bytes, double *kernel_seconds,
+                                    double *total_seconds) {
+  const double t0 = now_s();
+  if (!ctx || !c || !out_bytes) return fail(CODAG_ERR_INVALID, "brle_decode_host: null argument");
+  if (chunk_bytes == 0) chunk_bytes = 262144;  // brle.h:8
+  // [u32 n+1][u64 ptr[n+1]][u64 original_len][streams] (brle.h:281-296, 310-316)
+  if (cb < 4) return fail(CODAG_ERR_FORMAT, "brle container shorter than its header");
+  uint32_t n_ptr;
+  std::memcpy(&n_ptr, c, 4);
+  if (n_ptr == 0 || cb < 4 + 8ull * n_ptr + 8)
+    return fail(CODAG_ERR_FORMAT, "brle container shorter than its ptr array");
+  const uint64_t n = n_ptr - 1;
+  std::vector<uint64_t> ptr(n_ptr);
+  std::memcpy(ptr.data(), c + 4, 8ull * n_ptr);
+  uint64_t orig;
+  std::memcpy(&orig, c + 4 + 8ull * n_ptr, 8);
+  const uint64_t head = 4 + 8ull * n_ptr + 8;
+  const uint8_t *data = c + head;
+  const uint64_t data_bytes = cb - head;
+  if (ptr[0] != 0) return fail(CODAG_ERR_FORMAT, "brle ptr[0] is not 0");
+  for (uint64_t i = 0; i < n; i++)
+    if (ptr[i] > ptr[i + 1] || ptr[i + 1] > data_bytes)
+      return fail(CODAG_ERR_FORMAT, "chunk " + std::to_string(i) + " lies outside the stream");
+  if (orig > n * chunk_bytes || (n && orig <= (n - 1) * chunk_bytes))
+    return fail(CODAG_ERR_FORMAT, "original length does not match the chunk count");
+  if ((!out && orig) || out_cap < orig) return fail(CODAG_ERR_INVALID, "output buffer too small");
+  codag_status s = byte_codec_host(ctx, false, data, data_bytes, ptr.data(), n_ptr, n, chunk_bytes,
+                                   orig, out, out_bytes, kernel_seconds);
+  if (total_seconds) *total_seconds = now_s() - t0;
+  return s;
+}
+
+codag_status codag_lzss_decode_host(codag_ctx *ctx, const uint8_t *c, uint64_t cb, uint8_t *out,
+                                    uint64_t out_cap, uint64_t *out_bytes, double *kernel_seconds,
+                                    double *total_seconds) {
+  const double t0 = now_s();
+  if (!ctx || !c || !out_bytes) return fail(CODAG_ERR_INVALID, "lzss_decode_host: null argument");
+  // [u64 original_len][u32 chunk, n, hist, min, max][u64 ends[n]][streams]
+  // (lzss.h:1100-1119, read back at 1133-1146)
+  if (cb < 28) return fail(CODAG_ERR_FORMAT, "lzss container shorter than its header");
+  uint64_t orig;
+  uint32_t h[5];
+  std::memcpy(&orig, c, 8);
+  std::memcpy(h, c + 8, 20);
+  const uint64_t chunk_bytes = h[0], n = h[1];
+  if (h[2] != 2048 || h[3] != 3 || h[4] != 34)
+    return fail(CODAG_ERR_FORMAT, "lzss parameters other than history 2048, matches 3-34");
+  if (chunk_bytes == 0 && n) return fail(CODAG_ERR_FORMAT, "lzss chunk size is 0");
+  if (cb < 28 + 8 * n) return fail(CODAG_ERR_FORMAT, "lzss container shorter than its lens array");
+  std::vector<uint64_t> ends(n);
+  if (n) std::memcpy(ends.data(), c + 28, 8 * n);
+  const uint8_t *data = c + 28 + 8 * n;
+  const uint64_t data_bytes = cb - 28 - 8 * n;
+  for (uint64_t i = 0; i < n; i++)
+    if ((i && ends[i] < ends[i - 1]) || ends[i] > data_bytes)
+      return fail(CODAG_ERR_FORMAT, "chunk " + std::to_string(i) + " lies outside the stream");
+  if (orig > n * chunk_bytes || (n && orig <= (n - 1) * chunk_bytes))
+    return fail(CODAG_ERR_FORMAT, "original length does not match the chunk count");
+  if ((!out && orig) || out_cap < orig) return fail(CODAG_ERR_INVALID, "output buffer too small");
+  codag_status s = byte_codec_host(ctx, true, data, data_bytes, ends.data(), n, n,
+                                   chunk_bytes ? chunk_bytes : 131072, orig, out, out_bytes,
+                                   kernel_seconds);
   if (total_seconds) *total_seconds = now_s() - t0;
   return s;
 }

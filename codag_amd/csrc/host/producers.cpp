@@ -42,24 +42,6 @@ uint32_t v2_width_code(uint32_t bits) {
 
 static uint32_t bit_length(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
 
-static int run_threads(uint64_t n, int threads, const auto &fn) {
-  if (threads < 1) threads = (int)std::max(1u, std::thread::hardware_concurrency());
-  if ((uint64_t)threads > n) threads = (int)std::max<uint64_t>(1, n);
-  std::vector<std::thread> ts;
-  std::atomic<uint64_t> next{0};
-  auto body = [&]() {
-    for (;;) {
-      uint64_t i = next.fetch_add(1);
-      if (i >= n) break;
-      fn(i);
-    }
-  };
-  for (int t = 1; t < threads; t++) ts.emplace_back(body);
-  body();
-  for (auto &t : ts) t.join();
-  return threads;
-}
-
 // ------------------------------------------------------------------ RLE v1
 
 // Greedy ORC v1 chunk encoder: runs of 3..130 values with a constant signed

@@ -3,10 +3,14 @@
 
 #include "codag.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
+#include <vector>
 
 extern "C" codag_status codag_test_poll_shift(codag_ctx *ctx, uint32_t shift);
 
@@ -82,6 +86,26 @@ struct Rng {
   double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
   bool chance(double p) { return uniform() < p; }
 };
+
+// Runs fn(i) for i in [0, n) on `threads` host threads (< 1: all cores),
+// items handed out in order; returns the thread count used.
+template <typename Fn> int run_threads(uint64_t n, int threads, const Fn &fn) {
+  if (threads < 1) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  if ((uint64_t)threads > n) threads = (int)std::max<uint64_t>(1, n);
+  std::vector<std::thread> ts;
+  std::atomic<uint64_t> next{0};
+  auto body = [&]() {
+    for (;;) {
+      const uint64_t i = next.fetch_add(1);
+      if (i >= n) break;
+      fn(i);
+    }
+  };
+  for (int t = 1; t < threads; t++) ts.emplace_back(body);
+  body();
+  for (auto &t : ts) t.join();
+  return threads;
+}
 
 // RLE v2 width tables (include/rlev2/utils.h:28-59).
 extern const uint8_t kV2DecodeWidth[32];

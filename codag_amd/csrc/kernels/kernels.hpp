@@ -66,6 +66,14 @@ hipError_t launch_rlev2_pc(const uint8_t *comp, uint64_t comp_bytes, const uint6
                            uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t nexp,
                            uint32_t poll_shift, hipStream_t stream);
 
+// BRLE / LZSS, one wave64 per chunk (chunk_bytes < 2^31).
+hipError_t launch_brle_chunks(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *ptr,
+                              uint64_t n, uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,
+                              int32_t *status, hipStream_t stream);
+hipError_t launch_lzss_chunks(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *ends,
+                              uint64_t n, uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,
+                              int32_t *status, hipStream_t stream);
+
 // Plain device copy (16-byte loads/stores): the reference point for HBM
 // traffic counters.
 hipError_t launch_copy(uint8_t *dst, const uint8_t *src, uint64_t bytes, hipStream_t stream);

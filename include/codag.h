@@ -14,6 +14,10 @@
  *   rlev2::compress_gpu_orig      include/rlev2/encoder_trans.h:482-564
  *   deflate::decompress_gpu       include/zlib/zlib.h:1585-1587
  *   deflate_compressor.py         zlib_chunk_compress, deflate_compressor.py:44-126
+ *   brle::kernel_decompress       include/brle/brle.h:26-110 (and compress_gpu,
+ *                                 decompress_gpu 251-342)
+ *   lzss::kernel_decompress       include/lzss/lzss.h:234-305, 1014-1020 (and
+ *                                 compress_gpu, decompress_gpu_base 1060-1189)
  *   cuda_err_chk / PRINT_ERROR    include/common.h:50-64 (print-and-continue
  *                                 becomes a returned status)
  * The batch descriptor mirrors the shape of cuDF's gpu_inflate_input_s /
@@ -168,6 +172,39 @@ codag_status codag_inflate_batch_noalloc(const codag_stream_desc *d_descs, uint6
                                          codag_stream_result *d_results, uint32_t flags,
                                          uint32_t groups_per_cu, codag_stream_t stream);
 
+/* ------------------------------------------------------------- BRLE, LZSS */
+/* The reference's two byte-oriented codecs, which no CLI of the reference
+ * calls (include/brle/brle.h, include/lzss/lzss.h).
+ *
+ * Byte RLE: replaces brle::kernel_decompress (brle.h:26-110), one thread per
+ * chunk there. A header byte h > 127 is followed by h-127 literal bytes,
+ * h <= 127 by one byte repeated h+3 times. d_ptr: n_chunks+1 byte offsets
+ * into d_comp (the container's ptr array, brle.h:281-296); chunk i is
+ * [ptr[i], ptr[i+1]) and decodes into d_out + i*chunk_bytes, at most
+ * chunk_bytes (the reference's chunk is 262144 bytes, brle.h:8).
+ * d_out_len / d_chunk_status (nullable): bytes produced and CODAG_CHUNK_*
+ * per chunk (TRUNCATED: the stream ends inside an item; OVERFLOW: output
+ * past chunk_bytes; CORRUPT: the chunk lies outside comp_bytes or is
+ * >= 2^31 bytes). chunk_bytes < 2^31. */
+codag_status codag_brle_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
+                               const uint64_t *d_ptr, uint64_t n_chunks, uint64_t chunk_bytes,
+                               uint8_t *d_out, uint64_t *d_out_len, int32_t *d_chunk_status,
+                               codag_stream_t stream);
+
+/* LZSS with a 2048-byte history: replaces lzss::kernel_decompress
+ * (lzss.h:234-305, 1014-1020), one thread per chunk there. A flag byte
+ * covers the next 8 items, least significant bit first: 1 = literal byte,
+ * 0 = 2-byte little-endian v, length (v & 31)+3, offset v >> 5.
+ * d_ends: n_chunks cumulative stream ends (the container's lens array,
+ * lzss.h:1111, 1142); chunk i is [ends[i-1] (0 for i = 0), ends[i]) and
+ * decodes into d_out + i*chunk_bytes (131072 in the reference, lzss.h:21).
+ * Chunk codes as for BRLE, plus CORRUPT for an offset of 0 or one that
+ * reaches before the chunk's first byte. */
+codag_status codag_lzss_decode(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
+                               const uint64_t *d_ends, uint64_t n_chunks, uint64_t chunk_bytes,
+                               uint8_t *d_out, uint64_t *d_out_len, int32_t *d_chunk_status,
+                               codag_stream_t stream);
+
 /* ------------------------------------------------- host convenience (sync) */
 /* Used by the drop-in CLIs. Inputs are host buffers laid out exactly as the
  * reference's files; outputs are host buffers of at least the original
@@ -212,6 +249,28 @@ codag_status codag_inflate_chunks_host(codag_ctx *ctx, const uint8_t *comp, uint
                                        uint64_t chunk_bytes, uint32_t flags, uint8_t *out,
                                        uint64_t out_cap, uint64_t *out_bytes,
                                        double *kernel_seconds, double *total_seconds);
+
+/* BRLE container as brle::compress_gpu writes it (brle.h:281-296):
+ * [u32 n_chunks+1][u64 ptr[n_chunks+1]][u64 original_len][chunk streams],
+ * replacing brle::decompress_gpu (brle.h:305-342). chunk_bytes 0 means the
+ * reference's 262144. Each chunk must produce exactly its share of
+ * original_len (checked; the reference checks nothing). */
+codag_status codag_brle_decode_host(codag_ctx *ctx, const uint8_t *container,
+                                    uint64_t container_bytes, uint64_t chunk_bytes, uint8_t *out,
+                                    uint64_t out_cap, uint64_t *out_bytes, double *kernel_seconds,
+                                    double *total_seconds);
+
+/* LZSS container as lzss::compress_gpu writes it (lzss.h:1100-1119):
+ * [u64 original_len][u32 chunk_size, n_chunks, history 2048, min match 3,
+ * max match 34][u64 ends[n_chunks]][chunk streams], replacing
+ * lzss::decompress_gpu_base (lzss.h:1126-1189). Other history/match
+ * parameters are rejected (CODAG_ERR_FORMAT): the reference's decoder has
+ * them compiled in. Each chunk must produce exactly its share of
+ * original_len. */
+codag_status codag_lzss_decode_host(codag_ctx *ctx, const uint8_t *container,
+                                    uint64_t container_bytes, uint8_t *out, uint64_t out_cap,
+                                    uint64_t *out_bytes, double *kernel_seconds,
+                                    double *total_seconds);
 
 /* --------------------------------------------------- producers (CPU, C++) */
 /* Buffers returned by producers are allocated by the library with malloc;
@@ -260,6 +319,27 @@ codag_status codag_deflate_compress_chunks(const uint8_t *in, uint64_t in_bytes,
                                            uint64_t **blk_offset, uint64_t **col_len,
                                            uint64_t *n_chunks);
 
+/* BRLE container writer (role of brle::compress_gpu, brle.h:251-303). The
+ * chunk streams are the reference encoder's bytes (brle::encode,
+ * brle.h:129-230, on its direct path: brle.h:23 leaves USE_LOOKAHEAD
+ * undefined): literal groups of up to 128 bytes, runs of 3-130. chunk_bytes
+ * 0 means 262144. ptr[0] is written as 0 (the reference scans an entry its
+ * kernel never writes, brle.h:244-248, 267). */
+codag_status codag_brle_encode(const uint8_t *in, uint64_t in_bytes, uint64_t chunk_bytes,
+                               int threads, uint8_t **container, uint64_t *container_bytes);
+
+/* LZSS container writer (role of lzss::compress_gpu, lzss.h:1060-1124). The
+ * chunk streams are the reference encoder's bytes (compress_func and
+ * find_match, lzss.h:75-113, 310-419): at each position the longest match
+ * of 3-34 bytes that starts 1-2047 bytes back, lies wholly before the
+ * position and does not start at the oldest history byte; of equal lengths
+ * the oldest. Chunks are chunk_bytes (0 means 131072) with a shorter last
+ * chunk (the reference requires whole chunks, lzss.h:1065-1068). The flag
+ * byte the reference reserves after a full group at a chunk's end, which it
+ * leaves uninitialised, is 0 here. threads 0 = all cores. */
+codag_status codag_lzss_encode(const uint8_t *in, uint64_t in_bytes, uint64_t chunk_bytes,
+                               int threads, uint8_t **container, uint64_t *container_bytes);
+
 /* ------------------------------------------ synthetic workloads (CPU, C++) */
 /* Seeded generators of the benchmark columns/streams. Each returns the raw
  * data and the encoded form ready for decode. */
@@ -296,6 +376,14 @@ codag_status codag_deflate_compress_mixed(const uint8_t *in, uint64_t in_bytes,
                                           uint32_t pct_fixed, uint32_t pct_stored,
                                           uint64_t block_bytes, uint64_t seed, uint8_t **stream,
                                           uint64_t *stream_bytes, uint64_t counts[3]);
+
+/* Bytes for the BRLE / LZSS benchmarks. kind 0 (BRLE): runs of 3 +
+ * Geometric(mean 30) equal bytes, at most 130, alternating with literal
+ * stretches of 1-64 bytes of an order-1 Markov text. kind 1 (LZSS): text of
+ * words from a 512-word vocabulary (2-9 letters, skewed choice), spaces,
+ * numbers and newlines. */
+codag_status codag_synth_byte_payload(uint64_t n_bytes, uint32_t kind, uint64_t seed,
+                                      uint8_t **raw);
 
 /* ---------------------------------------------------------------- autotune */
 typedef enum codag_codec { CODAG_RLEV1 = 1, CODAG_RLEV2 = 2, CODAG_DEFLATE = 3 } codag_codec;

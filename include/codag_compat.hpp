@@ -10,6 +10,8 @@
  *   rlev2::compress_gpu_orig      include/rlev2/encoder_trans.h:482-564
  *   rlev2::decompress_gpu_orig    include/rlev2/decoder_trans.h:57-133
  *   deflate::decompress_gpu       include/zlib/zlib.h:1583-1700
+ *   brle::compress_gpu / decompress_gpu        include/brle/brle.h:251-342
+ *   lzss::compress_gpu / decompress_gpu_base   include/lzss/lzss.h:1060-1189
  *
  * Same template parameters, argument meaning, side files and stdout lines as
  * the reference. The launch-shape parameters (QUEUE_SIZE, NUM_CHUNK, NUM_WARP,
@@ -17,7 +19,9 @@
  * the reference prints them; the decode itself always runs the library's
  * tuned MI355X kernels. Output buffers are allocated the way each reference
  * caller releases them: rle_v1 and deflate outputs with malloc (rlev1.cu and
- * zlib.cu call free()), rlev2 outputs with new[] (rlev2.cu calls delete[]).
+ * zlib.cu call free()), rlev2 outputs with new[] (rlev2.cu calls delete[]);
+ * brle and lzss outputs with new[] as the reference allocates them (no
+ * reference program calls these two).
  * Where the reference prints an error and continues (common.h:50-64), these
  * print "Fatal Error: ..." to stderr and leave the outputs empty.
  *
@@ -240,5 +244,92 @@ void decompress_gpu(const uint8_t *const in, uint8_t **out, const uint64_t in_n_
 }
 
 }  // namespace deflate
+
+namespace codag_compat {
+
+/* Moves a library-allocated buffer into a new[] one. */
+inline uint8_t *to_new(uint8_t *p, uint64_t n) {
+  uint8_t *q = new uint8_t[n ? n : 1];
+  if (n) std::memcpy(q, p, n);
+  codag_host_free(p);
+  return q;
+}
+
+}  // namespace codag_compat
+
+namespace brle {
+
+/* brle.h:251-303: *out = [u32 n+1][u64 ptr[n+1]][u64 in bytes][streams]. */
+inline void compress_gpu(const uint8_t *in, uint8_t **out, const uint64_t in_n_bytes,
+                         uint64_t *out_n_bytes) {
+  *out = nullptr;
+  *out_n_bytes = 0;
+  uint8_t *c = nullptr;
+  uint64_t n = 0;
+  if (!codag_compat::report("brle::compress_gpu", codag_brle_encode(in, in_n_bytes, 0, 0, &c, &n)))
+    return;
+  *out = codag_compat::to_new(c, n);
+  *out_n_bytes = n;
+}
+
+/* brle.h:305-342: decodes a compress_gpu container into *out (new[]). */
+inline void decompress_gpu(const uint8_t *const in, uint8_t **out, const uint64_t in_n_bytes,
+                           uint64_t *out_n_bytes) {
+  *out = nullptr;
+  *out_n_bytes = 0;
+  codag_ctx *ctx = codag_compat::thread_ctx();
+  if (!ctx || in_n_bytes < 4) return;
+  uint32_t n_ptr = 0;
+  std::memcpy(&n_ptr, in, 4);
+  uint64_t orig = 0;
+  if (in_n_bytes >= 12 + 8ull * n_ptr) std::memcpy(&orig, in + 4 + 8ull * n_ptr, 8);
+  std::unique_ptr<uint8_t[]> buf(new uint8_t[orig ? orig : 1]);
+  uint64_t got = 0;
+  if (!codag_compat::report("brle::decompress_gpu",
+                            codag_brle_decode_host(ctx, in, in_n_bytes, 0, buf.get(), orig, &got,
+                                                   nullptr, nullptr)))
+    return;
+  *out = buf.release();
+  *out_n_bytes = got;
+}
+
+}  // namespace brle
+
+namespace lzss {
+
+/* lzss.h:1060-1124: *out = [u64 in bytes][u32 chunk, n, 2048, 3, 34][u64
+ * ends[n]][streams]. */
+inline void compress_gpu(const uint8_t *const in, uint8_t **out, const uint64_t in_n_bytes,
+                         uint64_t *out_n_bytes) {
+  *out = nullptr;
+  *out_n_bytes = 0;
+  uint8_t *c = nullptr;
+  uint64_t n = 0;
+  if (!codag_compat::report("lzss::compress_gpu", codag_lzss_encode(in, in_n_bytes, 0, 0, &c, &n)))
+    return;
+  *out = codag_compat::to_new(c, n);
+  *out_n_bytes = n;
+}
+
+/* lzss.h:1126-1189: decodes a compress_gpu container into *out (new[]). */
+inline void decompress_gpu_base(const uint8_t *const in, uint8_t **out, const uint64_t in_n_bytes,
+                                uint64_t *out_n_bytes) {
+  *out = nullptr;
+  *out_n_bytes = 0;
+  codag_ctx *ctx = codag_compat::thread_ctx();
+  if (!ctx || in_n_bytes < 8) return;
+  uint64_t orig = 0;
+  std::memcpy(&orig, in, 8);
+  std::unique_ptr<uint8_t[]> buf(new uint8_t[orig ? orig : 1]);
+  uint64_t got = 0;
+  if (!codag_compat::report("lzss::decompress_gpu_base",
+                            codag_lzss_decode_host(ctx, in, in_n_bytes, buf.get(), orig, &got,
+                                                   nullptr, nullptr)))
+    return;
+  *out = buf.release();
+  *out_n_bytes = got;
+}
+
+}  // namespace lzss
 
 #endif /* CODAG_COMPAT_HPP */

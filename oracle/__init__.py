@@ -12,6 +12,10 @@ It wraps ``liboracle.so`` (built from ``rle_oracle.c`` / ``zlib_oracle.c`` by
   tables of ``include/rlev2/utils.h:28-59``.
 * Deflate: the system zlib ``inflate()`` (RFC 1950/1951), which is the
   arithmetic ``include/zlib/zlib.h:807-866`` implements.
+* BRLE / LZSS (``byte_oracle.c``): restate ``include/brle/brle.h:26-100`` and
+  ``include/lzss/lzss.h:234-305`` (decoders) and ``brle.h:129-230``,
+  ``lzss.h:75-113, 310-419`` (encoders, statement by statement, with the
+  reference's rings and exhaustive match scan).
 
 Pinning (see tests/test_oracle.py): hand-derived known-answer vectors from the
 reference decoder text, the public ORC v2 examples, closed-form round trips
@@ -73,6 +77,14 @@ def lib():
         L.oracle_inflate_batch.argtypes = [_u8p, _u64p, _u64p, ctypes.c_uint64, _u8p, _u64p,
                                            _u64p, _u64p, _i32p, ctypes.c_int]
         L.oracle_inflate_batch.restype = ctypes.c_uint64
+        for fn in (L.oracle_brle_decode_chunk, L.oracle_lzss_decode_chunk):
+            fn.argtypes = [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64, _u64p]
+        for fn in (L.oracle_brle_decode_chunks, L.oracle_lzss_decode_chunks):
+            fn.argtypes = [_u8p, _u64p, ctypes.c_uint64, ctypes.c_uint64, _u8p, _u64p, _i32p]
+            fn.restype = ctypes.c_uint64
+        for fn in (L.oracle_brle_encode_chunk, L.oracle_lzss_encode_chunk):
+            fn.argtypes = [_u8p, ctypes.c_uint64, _u8p]
+            fn.restype = ctypes.c_uint64
         _LIB = L
     return _LIB
 
@@ -206,3 +218,64 @@ def rlev2_decode_width(code: int) -> int:
 
 def rlev2_closest_width(bits: int) -> int:
     return int(lib().oracle_rlev2_closest_width(bits))
+
+
+def _pad(b) -> np.ndarray:
+    d = _bytes(b)
+    return np.concatenate([d, np.zeros(1, np.uint8)])
+
+
+def brle_decode_chunk(stream, cap: int):
+    """(status, bytes) of one BRLE chunk stream decoded into at most cap bytes."""
+    src = _pad(stream)
+    out = np.zeros(max(cap, 1), dtype=np.uint8)
+    got = ctypes.c_uint64(0)
+    st = lib().oracle_brle_decode_chunk(_p(src), len(src) - 1, _p(out), cap, ctypes.byref(got))
+    return st, out[: got.value]
+
+
+def lzss_decode_chunk(stream, cap: int):
+    src = _pad(stream)
+    out = np.zeros(max(cap, 1), dtype=np.uint8)
+    got = ctypes.c_uint64(0)
+    st = lib().oracle_lzss_decode_chunk(_p(src), len(src) - 1, _p(out), cap, ctypes.byref(got))
+    return st, out[: got.value]
+
+
+def _byte_chunks(fn, data, idx, n, chunk_bytes, out):
+    data = _bytes(data)
+    if len(data) == 0:
+        data = np.zeros(1, np.uint8)
+    idx = np.ascontiguousarray(idx, dtype=np.uint64)
+    out = _outbuf(out, max(n * chunk_bytes, 1))
+    out_len = np.zeros(max(n, 1), dtype=np.uint64)
+    st = np.zeros(max(n, 1), dtype=np.int32)
+    fn(_p(data), _p(idx, _u64p), n, chunk_bytes, _p(out), _p(out_len, _u64p), _p(st, _i32p))
+    return st[:n], out_len[:n], out
+
+
+def brle_decode_chunks(data, ptr, chunk_bytes, out=None):
+    """ptr: n+1 offsets. Returns (status[n], out_len[n], out)."""
+    n = len(ptr) - 1
+    return _byte_chunks(lib().oracle_brle_decode_chunks, data, ptr, n, chunk_bytes, out)
+
+
+def lzss_decode_chunks(data, ends, chunk_bytes, out=None):
+    """ends: n cumulative stream ends. Returns (status[n], out_len[n], out)."""
+    return _byte_chunks(lib().oracle_lzss_decode_chunks, data, ends, len(ends), chunk_bytes, out)
+
+
+def brle_encode_chunk(raw) -> bytes:
+    src = _pad(raw)
+    n = len(src) - 1
+    out = np.zeros(n + n // 128 + 2, dtype=np.uint8)
+    k = lib().oracle_brle_encode_chunk(_p(src), n, _p(out))
+    return out[:k].tobytes()
+
+
+def lzss_encode_chunk(raw) -> bytes:
+    src = _pad(raw)
+    n = len(src) - 1
+    out = np.zeros(2 * n + n // 8 + 2, dtype=np.uint8)
+    k = lib().oracle_lzss_encode_chunk(_p(src), n, _p(out))
+    return out[:k].tobytes()

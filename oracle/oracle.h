@@ -100,6 +100,27 @@ uint64_t oracle_inflate_batch(const uint8_t *src, const uint64_t *src_off,
                               const uint64_t *dst_off, const uint64_t *dst_cap,
                               uint64_t *out_len, int32_t *status, int threads);
 
+/* ---- byte RLE and LZSS (byte_oracle.c) ----------------------------------
+ * BRLE restates include/brle/brle.h:26-100, LZSS include/lzss/lzss.h:234-305.
+ * *produced gets the bytes written; the return value is the chunk status. */
+int oracle_brle_decode_chunk(const uint8_t *in, uint64_t len, uint8_t *out, uint64_t cap,
+                             uint64_t *produced);
+int oracle_lzss_decode_chunk(const uint8_t *in, uint64_t len, uint8_t *out, uint64_t cap,
+                             uint64_t *produced);
+/* Whole containers: BRLE chunk i is data[ptr[i] .. ptr[i+1]); LZSS chunk i is
+ * data[ends[i-1] .. ends[i]) (ends[-1] = 0). Output chunk i at out + i*chunk. */
+uint64_t oracle_brle_decode_chunks(const uint8_t *data, const uint64_t *ptr, uint64_t n_chunks,
+                                   uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,
+                                   int32_t *status);
+uint64_t oracle_lzss_decode_chunks(const uint8_t *data, const uint64_t *ends, uint64_t n_chunks,
+                                   uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,
+                                   int32_t *status);
+/* One chunk's stream as the reference encoders write it (brle.h:129-230;
+ * lzss.h:75-113, 310-419); out needs n + n/128 + 2 (BRLE) or
+ * 2n + n/8 + 2 (LZSS) bytes. Returns the stream length. */
+uint64_t oracle_brle_encode_chunk(const uint8_t *in, uint64_t n, uint8_t *out);
+uint64_t oracle_lzss_encode_chunk(const uint8_t *in, uint64_t n, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

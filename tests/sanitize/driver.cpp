@@ -1,5 +1,5 @@
-// Sanitizer driver: the CPU producers (codag_amd/csrc/host/producers.cpp) and
-// the CPU oracle (oracle/*.c), built with AddressSanitizer and
+// Sanitizer driver: the CPU producers (codag_amd/csrc/host/producers.cpp,
+// byte_producers.cpp) and the CPU oracle (oracle/*.c), built with AddressSanitizer and
 // UndefinedBehaviorSanitizer, run over round trips, damaged and truncated
 // streams. Exit status 0 = every round trip exact and no sanitizer report
 // (a report aborts the run).
@@ -146,10 +146,77 @@ static void deflate_round_trips() {
   codag_host_free(mixed);
 }
 
+static void byte_codec_round_trips() {
+  for (uint32_t kind : {0u, 1u}) {
+    const uint64_t chunk = kind ? 5000 : 7000, n = 4 * chunk + 321;
+    uint8_t *raw = nullptr, *cont = nullptr;
+    uint64_t cont_n = 0;
+    EXPECT(codag_synth_byte_payload(n, kind, 17 + kind, &raw) == CODAG_OK);
+    if (kind == 0) {
+      EXPECT(codag_brle_encode(raw, n, chunk, 2, &cont, &cont_n) == CODAG_OK);
+      uint32_t n_ptr = 0;
+      std::memcpy(&n_ptr, cont, 4);
+      const uint64_t nc = n_ptr - 1;
+      std::vector<uint64_t> ptr(n_ptr);
+      std::memcpy(ptr.data(), cont + 4, 8 * n_ptr);
+      const uint8_t *data = cont + 12 + 8 * n_ptr;
+      std::vector<uint8_t> out(nc * chunk);
+      std::vector<uint64_t> len(nc);
+      std::vector<int32_t> st(nc);
+      EXPECT(oracle_brle_decode_chunks(data, ptr.data(), nc, chunk, out.data(), len.data(),
+                                       st.data()) == 0);
+      EXPECT(std::memcmp(out.data(), raw, n) == 0);
+      std::vector<uint8_t> ref(chunk + chunk / 128 + 2);
+      EXPECT(oracle_brle_encode_chunk(raw, chunk, ref.data()) == ptr[1] &&
+             std::memcmp(ref.data(), data, ptr[1]) == 0);
+      std::vector<uint8_t> s(data, data + ptr[1]), o(chunk);
+      uint64_t produced = 0;
+      for (uint64_t k = 0; k < s.size(); k += 3) {
+        std::vector<uint8_t> d = s;
+        d[k] ^= (uint8_t)(1 + rnd() % 255);
+        (void)oracle_brle_decode_chunk(d.data(), d.size(), o.data(), chunk, &produced);
+        EXPECT(produced <= chunk);
+        (void)oracle_brle_decode_chunk(d.data(), k, o.data(), chunk, &produced);
+        EXPECT(produced <= chunk);
+      }
+    } else {
+      EXPECT(codag_lzss_encode(raw, n, chunk, 2, &cont, &cont_n) == CODAG_OK);
+      uint32_t h[5];
+      std::memcpy(h, cont + 8, 20);
+      const uint64_t nc = h[1];
+      std::vector<uint64_t> ends(nc);
+      std::memcpy(ends.data(), cont + 28, 8 * nc);
+      const uint8_t *data = cont + 28 + 8 * nc;
+      std::vector<uint8_t> out(nc * chunk);
+      std::vector<uint64_t> len(nc);
+      std::vector<int32_t> st(nc);
+      EXPECT(oracle_lzss_decode_chunks(data, ends.data(), nc, chunk, out.data(), len.data(),
+                                       st.data()) == 0);
+      EXPECT(std::memcmp(out.data(), raw, n) == 0);
+      std::vector<uint8_t> ref(2 * chunk + chunk / 8 + 2);
+      EXPECT(oracle_lzss_encode_chunk(raw, chunk, ref.data()) == ends[0] &&
+             std::memcmp(ref.data(), data, ends[0]) == 0);
+      std::vector<uint8_t> s(data, data + ends[0]), o(chunk);
+      uint64_t produced = 0;
+      for (uint64_t k = 0; k < s.size(); k += 3) {
+        std::vector<uint8_t> d = s;
+        d[k] ^= (uint8_t)(1 + rnd() % 255);
+        (void)oracle_lzss_decode_chunk(d.data(), d.size(), o.data(), chunk, &produced);
+        EXPECT(produced <= chunk);
+        (void)oracle_lzss_decode_chunk(d.data(), k, o.data(), chunk, &produced);
+        EXPECT(produced <= chunk);
+      }
+    }
+    codag_host_free(raw);
+    codag_host_free(cont);
+  }
+}
+
 int main() {
   rlev1_round_trips();
   rlev2_round_trips();
   deflate_round_trips();
+  byte_codec_round_trips();
   std::printf("sanitizer driver: %d failures\n", failures);
   return failures ? 1 : 0;
 }

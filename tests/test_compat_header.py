@@ -3,8 +3,9 @@ of the container writer CLI.
 
 * A translation unit that makes the reference CLIs' own calls (template
   arguments and argument types of src/rlev1/rlev1.cu:168-193,
-  src/rlev2/rlev2.cu:55-110 and src/zlib/zlib.cu:102-104) compiles and links
-  against libcodag.so.
+  src/rlev2/rlev2.cu:55-110 and src/zlib/zlib.cu:102-104), and the BRLE /
+  LZSS host functions with the reference's signatures (brle.h:251,305;
+  lzss.h:1060,1126), compiles and links against libcodag.so.
 * deflate_compressor writes the files deflate_compressor.py:44-126 writes:
   zero-padded 128 KiB chunks, each zlib.compress(level=9)-ed and padded to 8
   bytes, [length][0][cumulative ends] offsets and padded lengths.
@@ -35,6 +36,10 @@ int main() {
   rlev2::decompress_gpu_orig<8, 48, uint32_t, 2, 4>(enc, enc_n, nch, blk_off, col_len, dec, dec_n, 131072);
   rlev2::decompress_gpu_orig<8, 64, uint32_t, 1, 16>(enc, enc_n, nch, blk_off, col_len, dec, dec_n, 131072);
   deflate::decompress_gpu<uint32_t, 262144 / 2, 64, 8, 1>(in, &out, 0, &n, col, 16, 131072);
+  brle::compress_gpu(in, &out, 0, &n);          // brle.h:251
+  brle::decompress_gpu(out, &out2, n, &n2);     // brle.h:305
+  lzss::compress_gpu(in, &out, 0, &n);          // lzss.h:1060
+  lzss::decompress_gpu_base(out, &out2, n, &n2);  // lzss.h:1126
   return 0;
 }
 """
