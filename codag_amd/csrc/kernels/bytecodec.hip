@@ -190,62 +190,69 @@ __device__ __forceinline__ int32_t brle_stream(const uint8_t *src, uint32_t len,
     const uint32_t lead = uni(w.lead);
     const uint32_t h0 = reinterpret_cast<const uint32_t *>(S.win)[lane];
     const uint32_t h1 = reinterpret_cast<const uint32_t *>(S.win)[kWave + lane];
-    // walk: item n's header index -> lane n of iq, its batch output offset
-    // -> lane n of ix
+    // walk (scalar): item n's header index -> lane n of iq. Only the header
+    // chain is followed here; sizes and limits are applied across lanes.
     const uint32_t qend = lead + (len - at);  // window index of the stream end
-    uint32_t q = lead, T = 0, n = 0, iq = 0, ix = 0;
-    while (n < (uint32_t)kWave && q < kHead && q < qend) {
+    const uint32_t qlim = min(qend, kHead);
+    uint32_t q = lead, n = 0, iq = 0;
+    while (n < (uint32_t)kWave && q < qlim) {
       const uint32_t h = head_byte(h0, h1, q);
-      const bool lit = h > 127u;
-      const uint32_t k = lit ? h - 127u : h + 3u;
-      const uint32_t inl = lit ? k + 1u : 2u;
-      if (q + inl > qend) {  // brle.h:79-86 read past the stream
-        err = kTrunc;
-        break;
-      }
-      if (o.pos + T + k > cap) {
-        err = kOverflow;
-        break;
-      }
-      if (T + k > kBrleBatch) break;
       iq = wrlane(iq, q, n);
-      ix = wrlane(ix, T, n);
       n++;
-      T += k;
-      q += inl;
+      q += h > 127u ? h - 126u : 2u;
     }
-    at += q - lead;
-    // the next window, while this batch is assembled
-    if (err == kOk && at < len) w = window_load(src, len, at, lane);
-    // assembly: lane i writes item i's first 16 bytes, 4 per step; the rest
-    // of longer items is written by the whole wave, one item at a time
+    // items (vector): lane i takes item i; the first item that reads past
+    // the stream (brle.h:79-86), passes the chunk or takes the batch past
+    // kBrleBatch ends the batch (the first two with their status)
     const bool act = lane < n;
     const uint32_t hq = act ? (uint32_t)S.win[iq] : 0u;
     const bool lit = hq > 127u;
     const uint32_t k = act ? (lit ? hq - 127u : hq + 3u) : 0u;
+    const uint32_t inl = lit ? k + 1u : 2u;
+    const uint32_t incl = wave_scan_dpp(k);
+    const uint32_t bad = !act ? 0u : (iq + inl > qend ? 1u : (o.pos + incl > cap ? 2u : 0u));
+    const uint64_t stop = __ballot(bad != 0u || (act && incl > kBrleBatch));
+    uint32_t nk = n;
+    if (stop) {
+      nk = (uint32_t)__builtin_ctzll(stop);
+      const uint32_t why = rdlane(bad, nk);
+      if (why) err = why == 1u ? kTrunc : kOverflow;
+      at += rdlane(iq, nk) - lead;
+    } else {
+      at += q - lead;
+    }
+    const uint32_t T = nk ? rdlane(incl, nk - 1u) : 0u;
+    // the next window, while this batch is assembled
+    if (err == kOk && at < len) w = window_load(src, len, at, lane);
+    // assembly: lane i writes item i into the ring, single bytes up to a
+    // 4-byte aligned ring address, then dwords (a run's byte repeated, or
+    // four literal bytes funnel-shifted out of two window dwords), then the
+    // tail bytes
+    const uint32_t kk = lane < nk ? k : 0u;
     const uint32_t s = iq + 1u;
-    const uint32_t c = act ? (uint32_t)S.win[s] : 0u;
-    const uint32_t x = o.pos + ix;
+    const uint32_t c = kk ? (uint32_t)S.win[s] : 0u;
+    const uint32_t r0 = o.pos + (incl - k) + o.mis;
+    const uint32_t hb = min((4u - (r0 & 3u)) & 3u, kk);
 #pragma unroll
-    for (uint32_t j = 0; j < 16u; j += 4u) {
-      if (!__ballot(j < k)) break;
-      if (j < k) {
-#pragma unroll
-        for (uint32_t t = 0; t < 4u; t++) {
-          if (j + t < k) S.ring[ridx(o, x + j + t)] = lit ? S.win[s + j + t] : (uint8_t)c;
-        }
+    for (uint32_t t = 0; t < 3u; t++)
+      if (t < hb) S.ring[(r0 + t) & kRingMask] = lit ? S.win[s + t] : (uint8_t)c;
+    const uint32_t nd = (kk - hb) >> 2;
+    const uint32_t c4 = c * 0x01010101u;
+    uint32_t *ring32 = reinterpret_cast<uint32_t *>(S.ring);
+    const uint32_t *win32 = reinterpret_cast<const uint32_t *>(S.win);
+    for (uint32_t d = 0; d < nd; d++) {
+      const uint32_t u = s + hb + 4u * d;
+      uint32_t v = c4;
+      if (lit) {
+        const uint32_t a = u >> 2;
+        v = __builtin_amdgcn_alignbit(win32[a + 1u], win32[a], (u & 3u) * 8u);
       }
+      ring32[((r0 + hb + 4u * d) & kRingMask) >> 2] = v;
     }
-    uint64_t lm = __ballot(k > 16u);
-    while (lm) {
-      const uint32_t i = (uint32_t)__builtin_ctzll(lm);
-      lm &= lm - 1;
-      const uint32_t xi = rdlane(x, i), ki = rdlane(k, i), si = rdlane(s, i), ci = rdlane(c, i);
-      const bool li = rdlane(hq, i) > 127u;
-#pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
-      for (uint32_t b = 16u + lane; b < ki; b += kWave)
-        S.ring[ridx(o, xi + b)] = li ? S.win[si + b] : (uint8_t)ci;
-    }
+    const uint32_t tb = hb + 4u * nd;
+#pragma unroll
+    for (uint32_t t = 0; t < 3u; t++)
+      if (tb + t < kk) S.ring[(r0 + tb + t) & kRingMask] = lit ? S.win[s + tb + t] : (uint8_t)c;
     o.pos += T;
     out_store(o, lane, o.pos, false);
     if (err != kOk) break;
