@@ -556,7 +556,8 @@ def bench_batch(ctx, args):
         "groups_per_cu": int(best.groups_per_cu), "wall_s": round(wall, 3),
         "per_file_launch_gbs": round(pf_gbs, 3),
         "per_file_sample": f"{len(samp)} streams, one inflate launch each",
-        "roofline": roofline("inflate_batch_kernel", ms, alg, "inflate_batch_kernel:c4"),
+        "roofline": roofline("inflate_batch_kernel", ms, alg,
+                             f"inflate_batch_kernel:c4:{ctx.info()[1] * best.groups_per_cu}"),
         "cpu_baseline": cpu,
     }
 
