@@ -475,8 +475,13 @@ def bench_batch(ctx, args):
     d_descs = ctx.upload(np.frombuffer(descs, np.uint8))
     rsz = 16
     d_res = ctx.alloc(n * rsz)
+    if args.c4_shape:
+        tpg, per = (int(x) for x in args.c4_shape.split(","))
+        ctx.tune_set(ca.DEFLATE, ca.TuneParams(tpg, 0, per, 1))
     best = ctx.tune_get(ca.DEFLATE)
-    log(f"[bench] c4: persistent launch, {best.groups_per_cu} waves per CU")
+    kname = "inflate_batch2_kernel" if best.threads_per_group == 128 else "inflate_batch_kernel"
+    log(f"[bench] c4: persistent launch, {best.threads_per_group} threads x "
+        f"{best.groups_per_cu} workgroups per CU")
 
     def check():
         rr = np.frombuffer(d_res.download(n * rsz).tobytes(), dtype=np.dtype(
@@ -553,11 +558,11 @@ def bench_batch(ctx, args):
         "dtype": "u8", "bit_exact": True, "streams": n,
         "decompressed_bytes": total, "compressed_bytes": int(slen.sum()),
         "blocks_fixed_dynamic_stored": [int(x) for x in mix], "crc32": crc,
-        "groups_per_cu": int(best.groups_per_cu), "wall_s": round(wall, 3),
+        "groups_per_cu": int(best.groups_per_cu), "threads_per_group": int(best.threads_per_group),
+        "wall_s": round(wall, 3),
         "per_file_launch_gbs": round(pf_gbs, 3),
         "per_file_sample": f"{len(samp)} streams, one inflate launch each",
-        "roofline": roofline("inflate_batch_kernel", ms, alg,
-                             f"inflate_batch_kernel:c4:{ctx.info()[1] * best.groups_per_cu}"),
+        "roofline": roofline(kname, ms, alg, f"{kname}:c4:{ctx.info()[1] * best.groups_per_cu}"),
         "cpu_baseline": cpu,
     }
 
@@ -719,6 +724,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="c1/c2: keep the default launch shape instead of tuning per column")
+    ap.add_argument("--c4-shape", default=None, metavar="THREADS,PER_CU",
+                    help="c4: batch launch shape (64 = one wave per stream, 128 = a decoding "
+                         "and a writing wave), workgroups per CU; default: the library's")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="first run the plain copy kernel over MIB MiB (traffic calibration)")
     args = ap.parse_args()

@@ -67,6 +67,7 @@ struct codag_ctx {
   uint64_t g_n = 0;
   const void *g_results = nullptr;
   uint32_t g_flags = 0, g_groups = 0;
+  bool g_two_wave = false;
   hipStream_t g_stream = nullptr;
   int g_slot = -1;
   uint32_t poll_shift = 0;  // RLE v2 queue kernel: nonzero = waits without polls (tests)
@@ -121,7 +122,7 @@ void default_tune(codag_ctx *c) {
   c->tune[CODAG_RLEV1] = {512, 2048, 0, 4};
   c->tune[CODAG_RLEV2] = {128, 4096, 0, 2};
   // 28 waves per CU: the inflate kernel's LDS and register budget allow seven per SIMD
-  c->tune[CODAG_DEFLATE] = {64, 0, 28, 1};
+  c->tune[CODAG_DEFLATE] = {128, 0, 12, 1};  // two-wave batch workgroups
   c->tune[0] = {0, 0, 0, 0};
 }
 
@@ -369,6 +370,7 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
   DeviceGuard g(ctx->device);
   hipStream_t s = pick(ctx, stream);
   const uint32_t per_cu = std::max(1u, ctx->tune[CODAG_DEFLATE].groups_per_cu);
+  const bool two_wave = ctx->tune[CODAG_DEFLATE].threads_per_group == 128;
   uint32_t groups = ctx->cu_count * per_cu;
   if (n && groups > n) groups = (uint32_t)n;
   if (groups == 0) groups = 1;
@@ -378,7 +380,7 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
   // Replay the captured graph when the same batch is launched again.
   if (ctx->graph_exec && ctx->g_descs == d_descs && ctx->g_n == n &&
       ctx->g_results == d_results && ctx->g_flags == flags && ctx->g_groups == groups &&
-      ctx->g_stream == s && ctx->g_slot == slot) {
+      ctx->g_two_wave == two_wave && ctx->g_stream == s && ctx->g_slot == slot) {
     HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
     HIP_TRY(hipEventRecord(ctx->slots[slot].done, s));
     return CODAG_OK;
@@ -395,7 +397,7 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
   // Capture counter reset + persistent kernel into a graph on this stream.
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t le = launch_inflate_batch(d_descs, n, d_results, flags, counter, groups, s);
+    hipError_t le = launch_inflate_batch(d_descs, n, d_results, flags, counter, groups, s, two_wave);
     e = hipStreamEndCapture(s, &ctx->graph);
     if (le != hipSuccess) e = le;
   }
@@ -406,6 +408,7 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
     ctx->g_results = d_results;
     ctx->g_flags = flags;
     ctx->g_groups = groups;
+    ctx->g_two_wave = two_wave;
     ctx->g_stream = s;
     ctx->g_slot = slot;
     HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
@@ -417,7 +420,7 @@ codag_status codag_inflate_batch(codag_ctx *ctx, const codag_stream_desc *d_desc
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   ctx->graph = nullptr;
   ctx->graph_exec = nullptr;
-  HIP_TRY(launch_inflate_batch(d_descs, n, d_results, flags, counter, groups, s));
+  HIP_TRY(launch_inflate_batch(d_descs, n, d_results, flags, counter, groups, s, two_wave));
   HIP_TRY(hipEventRecord(ctx->slots[slot].done, s));
   return CODAG_OK;
 }
@@ -796,6 +799,8 @@ codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_
     return fail(CODAG_ERR_INVALID, "tune_set: launch shape not compiled in");
   if (codec == CODAG_DEFLATE && (p->groups_per_cu == 0 || p->groups_per_cu > 64))
     return fail(CODAG_ERR_INVALID, "tune_set: groups_per_cu must be 1..64");
+  if (codec == CODAG_DEFLATE && p->threads_per_group != 64 && p->threads_per_group != 128)
+    return fail(CODAG_ERR_INVALID, "tune_set: Deflate threads_per_group must be 64 or 128");
   ctx->tune[codec] = *p;
   return CODAG_OK;
 }
@@ -812,6 +817,7 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
   std::vector<codag_tune_params> cands;
   if (codec == CODAG_DEFLATE) {
     for (uint32_t per : {8u, 16u, 20u, 24u, 28u}) cands.push_back({64, 0, per, 1});
+    for (uint32_t per : {8u, 12u, 14u}) cands.push_back({128, 0, per, 1});  // two-wave
   } else {
     if (codec == CODAG_RLEV1) cands.push_back({512, 2048, 0, 4});  // pipelined walker + helpers
     cands.push_back({256, 4096, 0, 2});  // walker + expander shapes

@@ -132,6 +132,17 @@ __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t j) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
 }
 
+// Orders this wave's LDS accesses around a point where lanes exchange data
+// through LDS. The decoder's tables, slots and ring are written and read by
+// one wave, so fences (no workgroup barrier) are enough, and a second wave
+// of the workgroup (the writing wave of the two-wave batch kernel) is never
+// waited for.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // ------------------------------------------------------------- bit reader
 
 struct Bits {
@@ -430,6 +441,41 @@ __device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t to
   return kOk;
 }
 
+// Copies ln stored-block bytes from src + at through the ring (RFC 1951
+// 3.2.4), 256 bytes per step.
+__device__ __forceinline__ int32_t store_copy(Out &o, const uint8_t *src, uint64_t at, uint32_t ln,
+                                              uint32_t lane) {
+  if (o.pos + ln > o.cap) return kOverflow;
+  for (uint32_t done = 0; done < ln;) {
+    const uint32_t step = (ln - done) < 256u ? (ln - done) : 256u;
+#pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
+    for (uint32_t i = lane; i < step; i += kWave) o.ring[ridx(o, o.pos + i)] = src[at + done + i];
+    o.pos += step;
+    done += step;
+    if (o.pos - o.flushed >= kFlushMin) out_store(o, lane, o.pos, false);
+  }
+  return kOk;
+}
+
+// The last partial flush, then the Adler-32 of the bytes written (a wave
+// reduction of the per-lane partial sums, each already < 65521).
+__device__ __forceinline__ uint32_t out_finish(Out &o, uint32_t lane) {
+  out_store(o, lane, o.pos, true);
+  uint32_t s1r = o.s1, s2r = o.s2;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    s1r += __shfl_xor(s1r, d, kWave);
+    s2r += __shfl_xor(s2r, d, kWave);
+  }
+  uint64_t s1 = s1r, s2 = s2r;
+  s1 %= 65521u;
+  s2 %= 65521u;
+  const uint64_t n = o.pos % 65521u;
+  const uint32_t A = (uint32_t)((1 + s1) % 65521u);
+  const uint32_t B = (uint32_t)((n * ((1 + s1) % 65521u) + 65521u * 65521u - s2) % 65521u);
+  return uni((B << 16) | A);
+}
+
 // ------------------------------------------------------------- table build
 
 __device__ __forceinline__ uint32_t make_entry(uint32_t tab, uint32_t sym, uint32_t len) {
@@ -460,12 +506,12 @@ __device__ __noinline__ int32_t build_table(const uint8_t *lens, uint32_t n, uin
   const uint32_t lane = lane_id();
   if (lane < 16) h.count[lane] = 0;
   for (uint32_t i = lane; i < (1u << bits); i += kWave) lut[i] = 0;
-  __syncthreads();
+  wave_sync();
   for (uint32_t s = lane; s < n; s += kWave) {
     const uint32_t L = lens[s];
     if (L) atomicAdd(reinterpret_cast<uint32_t *>(&h.count[0]) + (L >> 1), 1u << (16 * (L & 1)));
   }
-  __syncthreads();
+  wave_sync();
   // counts are packed two per dword above; all lanes walk them uniformly
   int32_t left = 1;
   uint32_t maxlen = 0, code = 0, idx = 0;
@@ -487,12 +533,12 @@ __device__ __noinline__ int32_t build_table(const uint8_t *lens, uint32_t n, uin
     if (tab == kTabCodes) return kCorrupt;
     if (maxlen > 1) return kCorrupt;
   }
-  __syncthreads();
+  wave_sync();
   if (lane >= 1 && lane < 16) {
     h.first[lane] = (uint16_t)first_l;
     h.index[lane] = (uint16_t)index_l;
   }
-  __syncthreads();
+  wave_sync();
   uint32_t run[16];
 #pragma unroll
   for (int L = 0; L < 16; L++) run[L] = 0;
@@ -516,7 +562,7 @@ __device__ __noinline__ int32_t build_table(const uint8_t *lens, uint32_t n, uin
       }
     }
   }
-  __syncthreads();
+  wave_sync();
   return kOk;
 }
 
@@ -552,12 +598,6 @@ __device__ __forceinline__ int32_t status_code(int32_t st, uint32_t flags) {
   return st == kOverflow ? 1 : (st == kTrunc ? 2 : -1);
 }
 
-struct Result {
-  int32_t status;
-  uint64_t bytes;
-  uint32_t adler;
-};
-
 __device__ __forceinline__ int32_t read_dynamic(Bits &b, Lds &S, uint32_t *nlen_out) {
   const uint32_t lane = lane_id();
   bits_refill(b);
@@ -581,7 +621,7 @@ __device__ __forceinline__ int32_t read_dynamic(Bits &b, Lds &S, uint32_t *nlen_
     else if (lane < ncode) v = (uint32_t)((c2 >> (3 * (lane - n1))) & 7);
     S.lens[kClOrder[lane]] = (uint8_t)v;
   }
-  __syncthreads();
+  wave_sync();
   int32_t st = (int32_t)uni((uint32_t)build_table(S.lens, 19, kTabCodes, S.hl, S.lit, kCodeBits));
   if (st != kOk) return st;
   // literal/length + distance code lengths with repeat codes 16/17/18
@@ -602,7 +642,7 @@ __device__ __forceinline__ int32_t read_dynamic(Bits &b, Lds &S, uint32_t *nlen_
     uint32_t rep, val = 0;
     if (sym == 16) {
       if (i == 0) return kCorrupt;  // zlib: "invalid bit length repeat"
-      __syncthreads();
+      wave_sync();
       val = uni(S.lens[i - 1]);
       rep = 3 + (uint32_t)(b.buf & 3);
       bits_drop(b, 2);
@@ -620,7 +660,7 @@ __device__ __forceinline__ int32_t read_dynamic(Bits &b, Lds &S, uint32_t *nlen_
     i += rep;
   }
   if (bits_past_end(b)) return kTrunc;
-  __syncthreads();
+  wave_sync();
   if (uni(S.lens[256]) == 0) return kCorrupt;  // zlib: "missing end-of-block"
   st = (int32_t)uni((uint32_t)build_table(S.lens, nlen, kTabLit, S.hl, S.lit, kLitBits));
   if (st != kOk) return st;
@@ -633,10 +673,10 @@ __device__ __forceinline__ void load_fixed(Lds &S) {
   const uint32_t lane = lane_id();
   for (uint32_t s = lane; s < 288; s += kWave)
     S.lens[s] = (uint8_t)(s < 144 ? 8 : (s < 256 ? 9 : (s < 280 ? 7 : 8)));
-  __syncthreads();
+  wave_sync();
   build_table(S.lens, 288, kTabLit, S.hl, S.lit, kLitBits);
   for (uint32_t s = lane; s < 32; s += kWave) S.lens[s] = 5;
-  __syncthreads();
+  wave_sync();
   build_table(S.lens, 32, kTabDist, S.hd, S.dist, kDistBits);
 }
 
@@ -834,8 +874,12 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
   return st;
 }
 
-// Decodes symbols until end-of-block. Returns a status.
-__device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, uint32_t *slot, uint32_t lane) {
+// Decodes symbols until end-of-block. Returns a status. The sink takes each
+// batch (SinkOne writes it on this wave, SinkQueue hands it to the writing
+// wave) and returns the first output error seen so far.
+template <class Sink>
+__device__ __forceinline__ int32_t decode_block(Bits &b, Sink &out, const Lds &S, uint32_t *slot,
+                                                uint32_t lane) {
   for (;;) {
     // A symbol reads at most 48 bits, so a batch of floor(left / 48) symbols
     // cannot pass the stream end; near the end batches shrink to one symbol,
@@ -851,12 +895,47 @@ __device__ __forceinline__ int32_t decode_block(Bits &b, Out &o, const Lds &S, u
     uint32_t tok = 0, nt = 0;
     const int32_t sa = decode_tokens(b, S, slot, lane, tok, nt, limit);
     if (bits_past_end(b)) return kTrunc;
-    const int32_t sb = out_tokens(o, lane, tok, nt);
+    const int32_t sb = out.tokens(lane, tok, nt);
     if (sb != kOk) return sb;
     if (sa == kEob) return kOk;
     if (sa != kOk) return sa;
   }
 }
+
+struct Result {
+  int32_t status;
+  uint64_t bytes;
+  uint32_t adler;
+};
+
+// Output of a stream decoded by one wave: the wave writes its own batches.
+struct SinkOne {
+  Out o;
+  __device__ __forceinline__ void init(uint8_t *dst, uint8_t *ring, uint64_t cap) {
+    o.dst = dst;
+    o.ring = ring;
+    o.cap = (uint32_t)cap;
+    o.pos = 0;
+    o.flushed = 0;
+    o.mis = (uint32_t)(reinterpret_cast<uintptr_t>(dst) & 15);
+    o.s1 = 0;
+    o.s2 = 0;
+  }
+  __device__ __forceinline__ int32_t tokens(uint32_t lane, uint32_t tok, uint32_t nt) {
+    return out_tokens(o, lane, tok, nt);
+  }
+  // stored block data: ln bytes from src + at (RFC 1951 3.2.4)
+  __device__ __forceinline__ int32_t stored(const uint8_t *src, uint64_t at, uint32_t ln,
+                                            uint32_t lane) {
+    return store_copy(o, src, at, ln, lane);
+  }
+  // the final flush; status st of the decode; fills bytes and Adler-32
+  __device__ __forceinline__ int32_t finish(uint32_t lane, int32_t st, Result &res) {
+    res.adler = out_finish(o, lane);
+    res.bytes = o.pos;
+    return st;
+  }
+};
 
 // ------------------------------------------------------------- CRC-32 (gzip)
 
@@ -909,7 +988,7 @@ __device__ __forceinline__ uint32_t crc_x8n(uint32_t n) {
 __device__ __forceinline__ uint32_t wave_crc32(const uint8_t *p, uint32_t n, uint32_t *tab) {
   const uint32_t lane = lane_id();
   // the bytes may have been stored by other lanes of this wave just before
-  __syncthreads();
+  wave_sync();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   for (uint32_t i = lane; i < 256; i += kWave) {
     uint32_t c = i;
@@ -917,14 +996,14 @@ __device__ __forceinline__ uint32_t wave_crc32(const uint8_t *p, uint32_t n, uin
     for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ kCrcPoly : c >> 1;
     tab[i] = c;
   }
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int t = 1; t < 4; t++) {
     for (uint32_t i = lane; i < 256; i += kWave) {
       const uint32_t c = tab[(t - 1) * 256 + i];
       tab[t * 256 + i] = (c >> 8) ^ tab[c & 0xff];
     }
-    __syncthreads();
+    wave_sync();
   }
   const uint32_t seg = ((n + kWave - 1) / kWave + 3) & ~3u;
   const uint32_t lo = min(lane * seg, n), hi = min(lo + seg, n);
@@ -945,7 +1024,7 @@ __device__ __forceinline__ uint32_t wave_crc32(const uint8_t *p, uint32_t n, uin
     const uint32_t cl = rdlane(mine, l), xl = rdlane(shift, l);
     total = crc_mulmod(xl, total) ^ cl;
   }
-  __syncthreads();
+  wave_sync();
   return uni(total);
 }
 
@@ -996,8 +1075,11 @@ __device__ __forceinline__ int64_t gzip_header(const uint8_t *src, uint64_t len,
   return (int64_t)at;
 }
 
-__device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t len, uint8_t *dst, uint64_t cap,
-                                 uint32_t flags, Lds &S) {
+// Decodes one zlib / raw / gzip stream; `out` takes the decoded batches
+// (SinkOne: this wave writes them; SinkQueue: a second wave does).
+template <class Sink>
+__device__ __forceinline__ Result inflate_stream_t(const uint8_t *src, uint64_t len, uint8_t *dst,
+                                                   uint64_t cap, uint32_t flags, Lds &S, Sink &out) {
   const uint32_t lane = lane_id();
   // the code lengths are dead while a block's symbols are decoded: their
   // bytes hold the window decoder's 64 token slots
@@ -1041,16 +1123,6 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
   }
   bits_seek(b, lead + start);
 
-  Out o;
-  o.dst = dst;
-  o.ring = S.ring;
-  o.cap = (uint32_t)cap;
-  o.pos = 0;
-  o.flushed = 0;
-  o.mis = (uint32_t)(reinterpret_cast<uintptr_t>(dst) & 15);
-  o.s1 = 0;
-  o.s2 = 0;
-
   int32_t st = kOk;
   bool last = false;
   uint32_t tables = 0;  // 1 = fixed tables loaded
@@ -1071,53 +1143,32 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
       if (ln != (~nln & 0xffff)) { st = kCorrupt; break; }
       const uint64_t at = bits_pos(b) / 8 - lead;  // stream byte offset of the data
       if (at + ln > len) { st = kTrunc; break; }
-      if (o.pos + ln > o.cap) { st = kOverflow; break; }
-      for (uint32_t done = 0; done < ln;) {  // through the ring, <= 256 bytes per step
-        const uint32_t step = (ln - done) < 256u ? (ln - done) : 256u;
-#pragma clang loop vectorize(disable) interleave(disable) unroll(disable)
-        for (uint32_t i = lane; i < step; i += kWave)
-          o.ring[ridx(o, o.pos + i)] = src[at + done + i];
-        o.pos += step;
-        done += step;
-        if (o.pos - o.flushed >= kFlushMin) out_store(o, lane, o.pos, false);
-      }
+      st = out.stored(src, at, ln, lane);
+      if (st != kOk) break;
       bits_seek(b, lead + at + ln);
     } else if (type == 1) {  // fixed Huffman
       if (tables != 1) {
         load_fixed(S);
         tables = 1;
       }
-      st = decode_block(b, o, S, slot, lane);
+      st = decode_block(b, out, S, slot, lane);
     } else if (type == 2) {  // dynamic Huffman
       uint32_t nlen = 0;
       tables = 0;
       st = read_dynamic(b, S, &nlen);
-      if (st == kOk) st = decode_block(b, o, S, slot, lane);
+      if (st == kOk) st = decode_block(b, out, S, slot, lane);
     } else {
       st = kCorrupt;  // zlib: "invalid block type"
     }
   }
-  // drain: the last partial flush
-  out_store(o, lane, o.pos, true);
-  // wave reduction of the Adler-32 partial sums (each already < 65521)
-  uint32_t s1r = o.s1, s2r = o.s2;
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    s1r += __shfl_xor(s1r, d, kWave);
-    s2r += __shfl_xor(s2r, d, kWave);
-  }
-  uint64_t s1 = s1r, s2 = s2r;
-  s1 %= 65521u;
-  s2 %= 65521u;
-  const uint64_t n = o.pos % 65521u;
-  const uint32_t A = (uint32_t)((1 + s1) % 65521u);
-  const uint32_t B = (uint32_t)((n * ((1 + s1) % 65521u) + 65521u * 65521u - s2) % 65521u);
-  res.adler = uni((B << 16) | A);
+  // the last flush, the bytes written and their Adler-32; the status becomes
+  // the first error in stream order of the decode and the output
+  st = out.finish(lane, st, res);
   if (gzip) {
     // RFC 1952 trailer on the next byte boundary: CRC-32 and ISIZE, both
     // little-endian; zlib checks the CRC first ("incorrect data check"),
     // then the length ("incorrect length check")
-    res.adler = wave_crc32(dst, o.pos, reinterpret_cast<uint32_t *>(S.ring));
+    res.adler = wave_crc32(dst, (uint32_t)res.bytes, reinterpret_cast<uint32_t *>(S.ring));
     if (st == kOk) {
       bits_drop(b, (uint32_t)((8 - (bits_pos(b) & 7)) & 7));
       const uint64_t at = bits_pos(b) / 8 - lead;
@@ -1129,7 +1180,7 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
         const uint32_t isize = uni(src[at + 4]) | (uni(src[at + 5]) << 8) |
                                (uni(src[at + 6]) << 16) | (uni(src[at + 7]) << 24);
         if (!(flags & CODAG_INFLATE_SKIP_ADLER) && crc != res.adler) st = kCorrupt;
-        else if (isize != o.pos) st = kCorrupt;
+        else if (isize != (uint32_t)res.bytes) st = kCorrupt;
       }
     }
   } else if (st == kOk && !raw) {
@@ -1145,8 +1196,14 @@ __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t le
     }
   }
   res.status = st;
-  res.bytes = o.pos;
   return res;
+}
+
+__device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t len, uint8_t *dst,
+                                                 uint64_t cap, uint32_t flags, Lds &S) {
+  SinkOne out;
+  out.init(dst, S.ring, cap > 0x7fffffffull ? 0x7fffffffull : cap);
+  return inflate_stream_t(src, len, dst, cap, flags, S, out);
 }
 
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
@@ -1198,6 +1255,199 @@ inflate_batch_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
   }
 }
 
+// ------------------------------------------------------ two-wave batch
+
+// A workgroup of two waves decodes one stream at a time: wave 0 decodes
+// (header, tables, symbols) and hands each batch of tokens, each stored
+// block and the end of the stream to wave 1 through a four-deep LDS queue;
+// wave 1 writes the output (ring copies, far gathers, flush, Adler-32). The
+// decoding wave's serial chain no longer waits for the output phase, which
+// bounds a large stream's time in an uneven batch. The two waves meet only
+// through the queue's head / tail / done words (release / acquire on LDS)
+// and at the top of each stream; every wait is bounded (kSpinLimit sleeps,
+// about a second), and one that runs out ends the stream as CORRUPT.
+constexpr uint32_t kQueue = 4;
+constexpr uint32_t kSpinLimit = 1u << 24;
+enum : uint32_t { kMsgTokens = 0, kMsgStored = 1, kMsgEnd = 2 };
+
+struct Msg {
+  uint32_t kind, nt, ln, pad;
+  uint64_t at;
+};
+
+struct Queue {
+  uint32_t tok[kQueue][kWave];
+  Msg msg[kQueue];
+  uint32_t head;    // messages published by the decoding wave
+  uint32_t tail;    // messages the writing wave has finished with
+  int32_t wstat;    // the writing wave's first error
+  uint32_t done;    // the writing wave has flushed the stream
+  uint32_t bytes, adler;
+  uint32_t stream;  // the workgroup's current stream index
+};
+
+__device__ __forceinline__ uint32_t lds_acquire(uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_release(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The decoding wave's side: batches go into the queue.
+struct SinkQueue {
+  Queue *q;
+  uint32_t head;
+  bool ended, stuck;
+  __device__ __forceinline__ int32_t push(uint32_t lane, uint32_t kind, uint32_t tok, uint32_t nt,
+                                          uint64_t at, uint32_t ln) {
+    for (uint32_t k = 0;; k++) {  // a free slot
+      if (head - uni(lds_acquire(&q->tail)) < kQueue) break;
+      if (k >= kSpinLimit) {
+        stuck = true;
+        return kCorrupt;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    const uint32_t sl = head % kQueue;
+    q->tok[sl][lane] = tok;
+    if (lane == 0) {
+      q->msg[sl].kind = kind;
+      q->msg[sl].nt = nt;
+      q->msg[sl].ln = ln;
+      q->msg[sl].at = at;
+    }
+    head++;
+    lds_release(&q->head, head);
+    return (int32_t)uni(lds_acquire(reinterpret_cast<uint32_t *>(&q->wstat)));
+  }
+  __device__ __forceinline__ int32_t tokens(uint32_t lane, uint32_t tok, uint32_t nt) {
+    if (nt == 0) return (int32_t)uni(lds_acquire(reinterpret_cast<uint32_t *>(&q->wstat)));
+    return push(lane, kMsgTokens, tok, nt, 0, 0);
+  }
+  __device__ __forceinline__ int32_t stored(const uint8_t *, uint64_t at, uint32_t ln, uint32_t lane) {
+    return push(lane, kMsgStored, 0, 0, at, ln);
+  }
+  // hands over the end of the stream and waits for the writing wave; the
+  // writer's first error precedes any later error of the decode
+  __device__ __forceinline__ int32_t finish(uint32_t lane, int32_t st, Result &res) {
+    ended = true;
+    res.bytes = 0;
+    res.adler = 1;
+    if (!stuck && push(lane, kMsgEnd, 0, 0, 0, 0) == kCorrupt && stuck) return kCorrupt;
+    if (stuck) return kCorrupt;
+    for (uint32_t k = 0;; k++) {
+      if (uni(lds_acquire(&q->done))) break;
+      if (k >= kSpinLimit) return kCorrupt;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    res.bytes = uni(q->bytes);
+    res.adler = uni(q->adler);
+    const int32_t ws = (int32_t)uni((uint32_t)q->wstat);
+    return ws != kOk ? ws : st;
+  }
+};
+
+// The writing wave's side: takes messages in order until the end of the
+// stream, then publishes the bytes written, their Adler-32 and its first
+// error. After an error it only drains the queue.
+__device__ __forceinline__ void inflate_write(Queue &q, uint8_t *ring, uint8_t *dst, uint64_t cap,
+                                              const uint8_t *src) {
+  const uint32_t lane = lane_id();
+  SinkOne w;
+  w.init(dst, ring, cap > 0x7fffffffull ? 0x7fffffffull : cap);
+  int32_t wst = kOk;
+  for (uint32_t t = 0;; t++) {
+    bool ready = false;
+    for (uint32_t k = 0; k < kSpinLimit; k++) {
+      if (uni(lds_acquire(&q.head)) > t) {
+        ready = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (!ready) {
+      if (wst == kOk) wst = kCorrupt;
+      break;
+    }
+    const uint32_t sl = t % kQueue;
+    const uint32_t kind = uni(q.msg[sl].kind);
+    if (kind == kMsgEnd) break;
+    if (wst == kOk) {
+      int32_t e;
+      if (kind == kMsgTokens) e = w.tokens(lane, q.tok[sl][lane], uni(q.msg[sl].nt));
+      else e = w.stored(src, q.msg[sl].at, uni(q.msg[sl].ln), lane);
+      if (e != kOk) {
+        wst = e;
+        lds_release(reinterpret_cast<uint32_t *>(&q.wstat), (uint32_t)e);
+      }
+    }
+    lds_release(&q.tail, t + 1);
+  }
+  Result r;
+  w.finish(lane, kOk, r);
+  q.bytes = (uint32_t)r.bytes;
+  q.adler = r.adler;
+  q.wstat = wst;
+  // the output is in memory before the decoding wave reads it back (gzip
+  // CRC-32)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  lds_release(&q.done, 1u);
+}
+
+struct Lds2 {
+  Lds dec;                       // the decoding wave's tables, slots and CRC space
+  alignas(16) uint8_t ring[kRing];  // the writing wave's output ring
+  Queue q;
+};
+
+__global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
+inflate_batch2_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
+                      codag_stream_result *__restrict__ results, uint32_t flags,
+                      uint32_t *__restrict__ counter) {
+  __shared__ Lds2 S;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
+  const uint32_t lane = lane_id();
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S.q.stream = atomicAdd(counter, 1u);
+      S.q.head = 0;
+      S.q.tail = 0;
+      S.q.wstat = kOk;
+      S.q.done = 0;
+    }
+    __syncthreads();
+    const uint32_t i = uni(S.q.stream);
+    if ((uint64_t)i >= n) break;
+    const codag_stream_desc d = descs[i];
+    const uint64_t big = d.dst_cap;
+    if (big >= (2u << 20)) __builtin_amdgcn_s_setprio(3);
+    else if (big >= (1u << 20)) __builtin_amdgcn_s_setprio(2);
+    else if (big >= (512u << 10)) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+    if (wave == 0) {
+      SinkQueue out;
+      out.q = &S.q;
+      out.head = 0;
+      out.ended = false;
+      out.stuck = false;
+      Result r = inflate_stream_t(d.src, d.src_bytes, d.dst, d.dst_cap, flags, S.dec, out);
+      if (!out.ended) {  // a header error before any output: release the writer
+        Result t;
+        const int32_t e = out.finish(lane, r.status, t);
+        if (r.status == kOk) r.status = e;
+      }
+      if (lane == 0) {
+        results[i].bytes_written = r.bytes;
+        results[i].status = status_code(r.status, flags);
+        results[i].adler32 = r.adler;
+      }
+    } else {
+      inflate_write(S.q, S.ring, d.dst, d.dst_cap, d.src);
+    }
+  }
+}
+
 __global__ void zero_counter_kernel(uint32_t *c) { *c = 0; }
 
 // Device memory of the allocation-free batch entry point: word 0 is its
@@ -1243,11 +1493,15 @@ hipError_t launch_inflate_chunks(const uint8_t *comp, uint64_t comp_bytes, const
 
 hipError_t launch_inflate_batch(const codag_stream_desc *descs, uint64_t n,
                                 codag_stream_result *results, uint32_t flags, uint32_t *counter,
-                                uint32_t groups, hipStream_t stream) {
+                                uint32_t groups, hipStream_t stream, bool two_wave) {
   hipLaunchKernelGGL(inflate::zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
   if (n == 0) return hipGetLastError();
-  hipLaunchKernelGGL(inflate::inflate_batch_kernel, dim3(groups), dim3(kWaveSize), 0, stream, descs,
-                     n, results, flags, counter);
+  if (two_wave)
+    hipLaunchKernelGGL(inflate::inflate_batch2_kernel, dim3(groups), dim3(2 * kWaveSize), 0, stream,
+                       descs, n, results, flags, counter);
+  else
+    hipLaunchKernelGGL(inflate::inflate_batch_kernel, dim3(groups), dim3(kWaveSize), 0, stream,
+                       descs, n, results, flags, counter);
   return hipGetLastError();
 }
 

@@ -35,10 +35,11 @@ hipError_t launch_inflate_chunks(const uint8_t *comp, uint64_t comp_bytes, const
                                  int32_t *status, uint32_t flags, hipStream_t stream);
 
 // Persistent batch launch: `groups` resident workgroups pull streams from
-// *counter (zeroed by a kernel launched first on the same stream).
+// *counter (zeroed by a kernel launched first on the same stream). two_wave:
+// workgroups of a decoding and a writing wave per stream; else one wave.
 hipError_t launch_inflate_batch(const codag_stream_desc *descs, uint64_t n,
                                 codag_stream_result *results, uint32_t flags, uint32_t *counter,
-                                uint32_t groups, hipStream_t stream);
+                                uint32_t groups, hipStream_t stream, bool two_wave = false);
 
 // Device address (on the current device) of the inflate kernels' static
 // 256-byte scratch array; its first word is the work-queue counter of the

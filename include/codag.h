@@ -392,12 +392,14 @@ typedef enum codag_codec { CODAG_RLEV1 = 1, CODAG_RLEV2 = 2, CODAG_DEFLATE = 3 }
  * selects the kernel family: 0 the per-chunk kernels (RLE v2 also 64 threads
  * = one wave per chunk, window 4096 or 8192), 2 the walker + expander kernel
  * (window 4096; 128, 192 (RLE v2 only) or 256 threads = 1, 2 or 3 expander
- * waves), 4 the RLE v1 pipeline (512 threads, window 2048). codag_tune_set
- * rejects shapes that are not compiled in. */
+ * waves), 4 the RLE v1 pipeline (512 threads, window 2048). For Deflate's
+ * batch launch (codag_inflate_batch), 64 threads = one wave per stream, 128
+ * = a decoding and a writing wave per stream. codag_tune_set rejects shapes
+ * that are not compiled in. */
 typedef struct codag_tune_params {
   uint32_t threads_per_group; /* workgroup size (64..1024, multiple of 64) */
   uint32_t window_bytes;      /* compressed bytes staged in LDS per window */
-  uint32_t groups_per_cu;     /* Deflate persistent launches: resident waves per CU */
+  uint32_t groups_per_cu;     /* Deflate persistent launches: resident workgroups per CU */
   uint32_t queue_depth;       /* RLE: kernel family (above); Deflate: 1 */
 } codag_tune_params;
 
@@ -407,9 +409,9 @@ codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_
  * the matching decode call; for DEFLATE d_blk_off are the n+1 offsets and
  * d_col_len is ignored) and keep the fastest whose output equals the first
  * candidate's. best_ms (nullable) gets its time. DEFLATE candidates differ in
- * waves per CU of the persistent launch, which has min(CUs x groups_per_cu,
- * n_chunks) waves: the sample needs more than CUs x 28 streams for them to
- * differ at all. */
+ * the batch launch's workgroup (one or two waves per stream) and workgroups
+ * per CU; the launch has min(CUs x groups_per_cu, n_chunks) workgroups, so
+ * the sample needs more than CUs x 28 streams for all of them to differ. */
 codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_comp,
                             uint64_t comp_bytes, const uint64_t *d_blk_off, const uint64_t *d_col_len,
                             uint64_t n_chunks, uint32_t elem_bytes, uint64_t chunk_bytes,
