@@ -304,6 +304,10 @@ def bench_deflate(ctx, args):
     d_len = ctx.alloc(8 * n)
     d_st = ctx.alloc(4 * n)
 
+    if args.c3_waves:
+        ca.check(ca.lib().codag_test_chunks_two_wave(ctx.h, 1 if args.c3_waves == 2 else 2),
+                 "test_chunks_two_wave")
+
     def launch():
         ctx.inflate_chunks(d_comp, len(comp), d_off, n, CH, d_out, d_len, d_st, 0)
 
@@ -727,6 +731,9 @@ def main():
     ap.add_argument("--c4-shape", default=None, metavar="THREADS,PER_CU",
                     help="c4: batch launch shape (64 = one wave per stream, 128 = a decoding "
                          "and a writing wave), workgroups per CU; default: the library's")
+    ap.add_argument("--c3-waves", type=int, default=0, choices=[0, 1, 2],
+                    help="c3: waves per chunk (2 = a decoding and a writing wave; "
+                         "0 = the library's choice by chunk count)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="first run the plain copy kernel over MIB MiB (traffic calibration)")
     args = ap.parse_args()

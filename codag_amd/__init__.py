@@ -130,6 +130,7 @@ def lib():
         _sig(L, "codag_inflate_chunks", st, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _u32, _vp])
         _sig(L, "codag_inflate_batch", st, [_vp, _vp, _u64, _vp, _u32, _vp])
         _sig(L, "codag_inflate_batch_noalloc", st, [_vp, _u64, _vp, _u32, _u32, _vp])
+        _sig(L, "codag_test_chunks_two_wave", st, [_vp, _u32])
         _sig(L, "codag_brle_decode", st, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp])
         _sig(L, "codag_lzss_decode", st, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp])
         dd = ctypes.POINTER(ctypes.c_double)

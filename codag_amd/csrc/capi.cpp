@@ -68,6 +68,7 @@ struct codag_ctx {
   const void *g_results = nullptr;
   uint32_t g_flags = 0, g_groups = 0;
   bool g_two_wave = false;
+  uint32_t chunks_two_wave = 0;
   hipStream_t g_stream = nullptr;
   int g_slot = -1;
   uint32_t poll_shift = 0;  // RLE v2 queue kernel: nonzero = waits without polls (tests)
@@ -288,8 +289,14 @@ codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_
   if (chunk_bytes == 0) return fail(CODAG_ERR_INVALID, "inflate_chunks: chunk_bytes is 0");
   if (n_chunks > 0x7fffffffull) return fail(CODAG_ERR_INVALID, "inflate_chunks: too many chunks");
   DeviceGuard g(ctx->device);
+  // Two-wave workgroups (a decoding and a writing wave per chunk) shorten
+  // each chunk's decode but halve the chunks in flight: they pay while every
+  // chunk is resident at once (up to 14 two-wave workgroups per CU), not
+  // when chunks queue for slots.
+  bool two_wave = n_chunks <= (uint64_t)ctx->cu_count * 14;
+  if (ctx->chunks_two_wave) two_wave = ctx->chunks_two_wave == 1;
   HIP_TRY(launch_inflate_chunks(d_comp, comp_bytes, d_offsets, n_chunks, chunk_bytes, d_out,
-                                d_out_len, d_status, flags, pick(ctx, stream)));
+                                d_out_len, d_status, flags, pick(ctx, stream), two_wave));
   return CODAG_OK;
 }
 
@@ -772,6 +779,14 @@ codag_status codag_lzss_decode_host(codag_ctx *ctx, const uint8_t *c, uint64_t c
 // `shift` nonzero switches the RLE v2 walker + expander kernel to its
 // variant without polls in its bounded waits (0 restores the normal one), so
 // a test can drive the waits' error path.
+// Test hook, not in the public header: codag_inflate_chunks' workgroups --
+// 0 chosen by chunk count, 1 two waves per chunk, 2 one wave per chunk.
+codag_status codag_test_chunks_two_wave(codag_ctx *ctx, uint32_t on) {
+  if (!ctx) return fail(CODAG_ERR_INVALID, "test_chunks_two_wave: null ctx");
+  ctx->chunks_two_wave = on;
+  return CODAG_OK;
+}
+
 codag_status codag_test_poll_shift(codag_ctx *ctx, uint32_t shift) {
   if (!ctx) return fail(CODAG_ERR_INVALID, "test_poll_shift: null ctx");
   ctx->poll_shift = shift;

@@ -13,6 +13,7 @@
 #include <vector>
 
 extern "C" codag_status codag_test_poll_shift(codag_ctx *ctx, uint32_t shift);
+extern "C" codag_status codag_test_chunks_two_wave(codag_ctx *ctx, uint32_t on);
 
 namespace codag {
 

@@ -1265,7 +1265,8 @@ inflate_batch_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
 // bounds a large stream's time in an uneven batch. The two waves meet only
 // through the queue's head / tail / done words (release / acquire on LDS)
 // and at the top of each stream; every wait is bounded (kSpinLimit sleeps,
-// about a second), and one that runs out ends the stream as CORRUPT.
+// between half a second and two), and one that runs out ends the stream as
+// CORRUPT.
 constexpr uint32_t kQueue = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;
 enum : uint32_t { kMsgTokens = 0, kMsgStored = 1, kMsgEnd = 2 };
@@ -1363,7 +1364,9 @@ __device__ __forceinline__ void inflate_write(Queue &q, uint8_t *ring, uint8_t *
         ready = true;
         break;
       }
-      __builtin_amdgcn_s_sleep(1);
+      // an idle writer polls every ~200 cycles: a batch takes the decoder
+      // thousands, and fewer polls leave the CU's issue slots to decoders
+      __builtin_amdgcn_s_sleep(3);
     }
     if (!ready) {
       if (wst == kOk) wst = kCorrupt;
@@ -1448,6 +1451,52 @@ inflate_batch2_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
   }
 }
 
+// The chunk container decoded by two-wave workgroups, one chunk each.
+__global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
+inflate_chunks2_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
+                       const uint64_t *__restrict__ off, uint64_t n, uint64_t chunk_bytes,
+                       uint8_t *__restrict__ out, uint64_t *__restrict__ out_len,
+                       int32_t *__restrict__ status, uint32_t flags) {
+  __shared__ Lds2 S;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
+  const uint32_t lane = lane_id();
+  const uint64_t c = blockIdx.x;
+  const uint64_t o0 = off[c], o1 = off[c + 1];
+  if (o1 < o0 || o1 > comp_bytes) {  // no stream: neither wave decodes
+    if (threadIdx.x == 0) {
+      if (status) status[c] = status_code(kCorrupt, flags);
+      if (out_len) out_len[c] = 0;
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    S.q.head = 0;
+    S.q.tail = 0;
+    S.q.wstat = kOk;
+    S.q.done = 0;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    SinkQueue sink;
+    sink.q = &S.q;
+    sink.head = 0;
+    sink.ended = false;
+    sink.stuck = false;
+    Result r = inflate_stream_t(comp + o0, o1 - o0, out + c * chunk_bytes, chunk_bytes, flags, S.dec, sink);
+    if (!sink.ended) {
+      Result t;
+      const int32_t e = sink.finish(lane, r.status, t);
+      if (r.status == kOk) r.status = e;
+    }
+    if (lane == 0) {
+      if (status) status[c] = status_code(r.status, flags);
+      if (out_len) out_len[c] = r.bytes;
+    }
+  } else {
+    inflate_write(S.q, S.ring, out + c * chunk_bytes, chunk_bytes, comp + o0);
+  }
+}
+
 __global__ void zero_counter_kernel(uint32_t *c) { *c = 0; }
 
 // Device memory of the allocation-free batch entry point: word 0 is its
@@ -1484,10 +1533,15 @@ hipError_t inflate_scratch_ptr(uint32_t **out) {
 hipError_t launch_inflate_chunks(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *off,
                                  uint64_t n,
                                  uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,
-                                 int32_t *status, uint32_t flags, hipStream_t stream) {
+                                 int32_t *status, uint32_t flags, hipStream_t stream,
+                                 bool two_wave) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(inflate::inflate_chunks_kernel, dim3((uint32_t)n), dim3(kWaveSize), 0, stream,
-                     comp, comp_bytes, off, n, chunk_bytes, out, out_len, status, flags);
+  if (two_wave)
+    hipLaunchKernelGGL(inflate::inflate_chunks2_kernel, dim3((uint32_t)n), dim3(2 * kWaveSize), 0,
+                       stream, comp, comp_bytes, off, n, chunk_bytes, out, out_len, status, flags);
+  else
+    hipLaunchKernelGGL(inflate::inflate_chunks_kernel, dim3((uint32_t)n), dim3(kWaveSize), 0,
+                       stream, comp, comp_bytes, off, n, chunk_bytes, out, out_len, status, flags);
   return hipGetLastError();
 }
 

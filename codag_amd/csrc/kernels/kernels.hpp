@@ -29,10 +29,12 @@ hipError_t launch_rlev2_wave(const uint8_t *comp, uint64_t comp_bytes, const uin
                              uint64_t chunk_bytes, uint8_t *out, int32_t *status, uint32_t win,
                              hipStream_t stream);
 
+// two_wave: a decoding and a writing wave per chunk; else one wave.
 hipError_t launch_inflate_chunks(const uint8_t *comp, uint64_t comp_bytes, const uint64_t *off,
                                  uint64_t n,
                                  uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,
-                                 int32_t *status, uint32_t flags, hipStream_t stream);
+                                 int32_t *status, uint32_t flags, hipStream_t stream,
+                                 bool two_wave = false);
 
 // Persistent batch launch: `groups` resident workgroups pull streams from
 // *counter (zeroed by a kernel launched first on the same stream). two_wave:

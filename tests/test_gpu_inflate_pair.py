@@ -138,3 +138,27 @@ def test_pair_graph_replay(ctx):
             _check_batch(b.results(), raws)
     finally:
         ctx.tune_set(ca.DEFLATE, ONE)
+
+
+def test_pair_chunk_container(ctx):
+    say("two-wave chunk kernel: chunk container against the one-wave kernel and zlib")
+    from tests.gpu_helpers import inflate
+    from tests.test_gpu_inflate import container
+    raw = ca.synth_deflate_payload(24 * CH, CH, seed=23).tobytes()
+    chunks = [z(raw[i * CH:(i + 1) * CH], 6) for i in range(24)]
+    chunks[5] = chunks[5][:-9]                     # truncated
+    chunks[9] = bytes([0x78, 0x9c, 0xff]) + chunks[9][3:]  # corrupt
+    comp, off = container(chunks)
+    try:
+        ca.check(ca.lib().codag_test_chunks_two_wave(ctx.h, 2), "test_chunks_two_wave")
+        a = inflate(ctx, comp, off, CH)  # one wave per chunk
+        ca.check(ca.lib().codag_test_chunks_two_wave(ctx.h, 1), "test_chunks_two_wave")
+        b = inflate(ctx, comp, off, CH)  # two waves per chunk
+    finally:
+        ca.check(ca.lib().codag_test_chunks_two_wave(ctx.h, 0), "test_chunks_two_wave")
+    assert a[0].tolist() == b[0].tolist() and a[1].tolist() == b[1].tolist()
+    for i in range(24):
+        n = int(a[1][i])
+        assert np.array_equal(a[2][i * CH: i * CH + n], b[2][i * CH: i * CH + n]), i
+        rc, ref = oracle.zlib_inflate(comp[int(off[i]): int(off[i + 1])].tobytes(), CH)
+        assert b[0][i] == rc, i
