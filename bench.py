@@ -378,6 +378,7 @@ def bench_bytecodec(ctx, args, kind):
     d_len = ctx.alloc(8 * n)
     d_st = ctx.alloc(4 * n)
     fn = ctx.lzss_decode if lz else ctx.brle_decode
+    kname = "lzss_chunks_kernel" if lz else "brle_chunks2_kernel"
 
     def launch():
         fn(d_comp, len(data), d_idx, n, chunk, d_out, d_len, d_st)
@@ -437,7 +438,7 @@ def bench_bytecodec(ctx, args, kind):
         "value": round(n_bytes / (ms / 1e3) / 1e9, 2), "unit": "GB/s", "ms_per_step": round(ms, 4),
         "dtype": "u8", "bit_exact": True, "chunks": int(n), "compressed_bytes": int(len(data)),
         "decompressed_bytes": n_bytes, "crc32": crc,
-        "roofline": roofline(f"{kind}_chunks_kernel", ms, alg, f"{kind}_chunks_kernel:{kind}:{n}"),
+        "roofline": roofline(kname, ms, alg, f"{kname}:{kind}:{n}"),
         "cpu_baseline": cpu,
     }
 

@@ -14,13 +14,14 @@
 //         offset bytes back. Chunk i is data[end[i-1] .. end[i]) and decodes
 //         into out + i * chunk (128 KiB, lzss.h:21).
 //
-// One wave64 per chunk. Both formats are sequences of byte-aligned items
-// whose sizes follow from one header byte (BRLE) or one flag byte per eight
-// items (LZSS), so a batch is found by a scalar walk over header bytes that
-// sit in two VGPRs (v_readlane, no memory round trip per step), each lane
-// then takes one item, and the batch's output is assembled in an LDS ring
-// and stored to HBM as 16-byte aligned vectors. The next batch's input window
-// is loaded while the current batch is assembled.
+// Both formats are sequences of byte-aligned items whose sizes follow from
+// one header byte (BRLE) or one flag byte per eight items (LZSS), so a batch
+// is found by a scalar walk over header bytes that sit in two VGPRs
+// (v_readlane, no memory round trip per step), each lane then takes one
+// item, and the batch's output is assembled in an LDS ring and stored to HBM
+// as 16-byte aligned vectors. The next batch's input window is loaded while
+// the current batch is assembled. LZSS runs one wave64 per chunk; BRLE two,
+// the walking wave one batch ahead of the writing wave.
 //
 // The reference checks nothing; these kernels report per chunk: TRUNCATED if
 // the stream ends inside an item, OVERFLOW if the output would pass the
@@ -164,15 +165,54 @@ __device__ __forceinline__ void chunk_bounds(const uint64_t lo, const uint64_t h
 
 // ------------------------------------------------------------------ BRLE
 
-struct __align__(16) BrleLds {
+// Two waves per chunk, in lockstep (one workgroup barrier per batch): wave 0
+// walks batch i (window staging, header chain, item checks, output offsets)
+// while wave 1 writes batch i-1 into the ring and stores it to HBM. The walk
+// is scalar-bound and the writing vector/LDS-bound, so the two overlap, and a
+// 4096-chunk launch fills all 32 wave slots of a CU instead of 16. Batch i
+// uses window buffer i % 3 (the writer still reads i-1's literal bytes while
+// the walker stages i) and descriptor buffer i % 2.
+struct __align__(16) BrleLds2 {
   uint8_t ring[kRing];
-  uint8_t win[kWin];
+  uint8_t win[3][kWin];
+  uint32_t item[2][kWave];  // header index | batch output offset << 16
+  uint32_t nk[2];           // items in the batch
+  uint32_t base[2];         // output position of the batch's first byte
+  uint32_t last;            // the walker's last batch (kNoLast while walking)
 };
+constexpr uint32_t kNoLast = 0xffffffffu;
 
-// Decodes one BRLE stream of `len` bytes into dst (at most cap bytes).
-__device__ __forceinline__ int32_t brle_stream(const uint8_t *src, uint32_t len, uint8_t *dst,
-                                               uint32_t cap, BrleLds &S, uint32_t lane,
-                                               uint32_t *produced) {
+__global__ void __launch_bounds__(2 * kWave) brle_chunks2_kernel(const uint8_t *comp,
+                                                                 uint64_t comp_bytes,
+                                                                 const uint64_t *ptr, uint64_t n,
+                                                                 uint64_t chunk_bytes, uint8_t *out,
+                                                                 uint64_t *out_len, int32_t *status) {
+  __shared__ BrleLds2 S;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;
+  const uint64_t ci = blockIdx.x;
+  if (ci >= n) return;
+  uint64_t len64;
+  int32_t err;
+  chunk_bounds(ptr[ci], ptr[ci + 1], comp_bytes, len64, err);
+  if (err != kOk) {  // both waves leave before any barrier
+    if (threadIdx.x == 0) {
+      if (out_len) out_len[ci] = 0;
+      if (status) status[ci] = err;
+    }
+    return;
+  }
+  const uint8_t *src = comp + ptr[ci];
+  const uint32_t len = (uint32_t)len64;
+  uint8_t *dst = out + ci * chunk_bytes;
+  const uint32_t cap = (uint32_t)chunk_bytes;
+  if (threadIdx.x == 0) S.last = kNoLast;
+  __syncthreads();
+  // walker state
+  uint32_t at = 0, pos = 0;
+  Window w;
+  if (wave == 0) w = window_load(src, len, 0, lane);
+  // writer state
   Out o;
   o.dst = dst;
   o.ring = S.ring;
@@ -180,106 +220,100 @@ __device__ __forceinline__ int32_t brle_stream(const uint8_t *src, uint32_t len,
   o.pos = 0;
   o.flushed = 0;
   o.mis = (uint32_t)(reinterpret_cast<uintptr_t>(dst) & 15u);
-  int32_t err = kOk;
-  uint32_t at = 0;  // first unconsumed stream byte
-  Window w = window_load(src, len, 0, lane);
-  while (at < len) {
-    __syncthreads();
-    reinterpret_cast<uint4 *>(S.win)[lane] = w.v;
-    __syncthreads();
-    const uint32_t lead = uni(w.lead);
-    const uint32_t h0 = reinterpret_cast<const uint32_t *>(S.win)[lane];
-    const uint32_t h1 = reinterpret_cast<const uint32_t *>(S.win)[kWave + lane];
-    // walk (scalar): item n's header index -> lane n of iq. Only the header
-    // chain is followed here; sizes and limits are applied across lanes.
-    const uint32_t qend = lead + (len - at);  // window index of the stream end
-    const uint32_t qlim = min(qend, kHead);
-    uint32_t q = lead, n = 0, iq = 0;
-    while (n < (uint32_t)kWave && q < qlim) {
-      const uint32_t h = head_byte(h0, h1, q);
-      iq = wrlane(iq, q, n);
-      n++;
-      q += h > 127u ? h - 126u : 2u;
-    }
-    // items (vector): lane i takes item i; the first item that reads past
-    // the stream (brle.h:79-86), passes the chunk or takes the batch past
-    // kBrleBatch ends the batch (the first two with their status)
-    const bool act = lane < n;
-    const uint32_t hq = act ? (uint32_t)S.win[iq] : 0u;
-    const bool lit = hq > 127u;
-    const uint32_t k = act ? (lit ? hq - 127u : hq + 3u) : 0u;
-    const uint32_t inl = lit ? k + 1u : 2u;
-    const uint32_t incl = wave_scan_dpp(k);
-    const uint32_t bad = !act ? 0u : (iq + inl > qend ? 1u : (o.pos + incl > cap ? 2u : 0u));
-    const uint64_t stop = __ballot(bad != 0u || (act && incl > kBrleBatch));
-    uint32_t nk = n;
-    if (stop) {
-      nk = (uint32_t)__builtin_ctzll(stop);
-      const uint32_t why = rdlane(bad, nk);
-      if (why) err = why == 1u ? kTrunc : kOverflow;
-      at += rdlane(iq, nk) - lead;
-    } else {
-      at += q - lead;
-    }
-    const uint32_t T = nk ? rdlane(incl, nk - 1u) : 0u;
-    // the next window, while this batch is assembled
-    if (err == kOk && at < len) w = window_load(src, len, at, lane);
-    // assembly: lane i writes item i into the ring, single bytes up to a
-    // 4-byte aligned ring address, then dwords (a run's byte repeated, or
-    // four literal bytes funnel-shifted out of two window dwords), then the
-    // tail bytes
-    const uint32_t kk = lane < nk ? k : 0u;
-    const uint32_t s = iq + 1u;
-    const uint32_t c = kk ? (uint32_t)S.win[s] : 0u;
-    const uint32_t r0 = o.pos + (incl - k) + o.mis;
-    const uint32_t hb = min((4u - (r0 & 3u)) & 3u, kk);
-#pragma unroll
-    for (uint32_t t = 0; t < 3u; t++)
-      if (t < hb) S.ring[(r0 + t) & kRingMask] = lit ? S.win[s + t] : (uint8_t)c;
-    const uint32_t nd = (kk - hb) >> 2;
-    const uint32_t c4 = c * 0x01010101u;
-    uint32_t *ring32 = reinterpret_cast<uint32_t *>(S.ring);
-    const uint32_t *win32 = reinterpret_cast<const uint32_t *>(S.win);
-    for (uint32_t d = 0; d < nd; d++) {
-      const uint32_t u = s + hb + 4u * d;
-      uint32_t v = c4;
-      if (lit) {
-        const uint32_t a = u >> 2;
-        v = __builtin_amdgcn_alignbit(win32[a + 1u], win32[a], (u & 3u) * 8u);
+  for (uint32_t i = 0;; i++) {
+    if (wave == 0 && uni(S.last) == kNoLast) {
+      uint8_t *win = S.win[i % 3];
+      reinterpret_cast<uint4 *>(win)[lane] = w.v;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const uint32_t lead = uni(w.lead);
+      const uint32_t h0 = reinterpret_cast<const uint32_t *>(win)[lane];
+      const uint32_t h1 = reinterpret_cast<const uint32_t *>(win)[kWave + lane];
+      const uint32_t qend = lead + (len - at);
+      const uint32_t qlim = min(qend, kHead);
+      uint32_t q = lead, nw = 0, iq = 0;
+      while (nw < (uint32_t)kWave && q < qlim) {
+        const uint32_t h = head_byte(h0, h1, q);
+        iq = wrlane(iq, q, nw);
+        nw++;
+        q += h > 127u ? h - 126u : 2u;
       }
-      ring32[((r0 + hb + 4u * d) & kRingMask) >> 2] = v;
-    }
-    const uint32_t tb = hb + 4u * nd;
+      const bool act = lane < nw;
+      const uint32_t hq = act ? (uint32_t)win[iq] : 0u;
+      const bool lit = hq > 127u;
+      const uint32_t k = act ? (lit ? hq - 127u : hq + 3u) : 0u;
+      const uint32_t inl = lit ? k + 1u : 2u;
+      const uint32_t incl = wave_scan_dpp(k);
+      const uint32_t bad = !act ? 0u : (iq + inl > qend ? 1u : (pos + incl > cap ? 2u : 0u));
+      const uint64_t stop = __ballot(bad != 0u || (act && incl > kBrleBatch));
+      uint32_t nk = nw;
+      if (stop) {
+        nk = (uint32_t)__builtin_ctzll(stop);
+        const uint32_t why = rdlane(bad, nk);
+        if (why) err = why == 1u ? kTrunc : kOverflow;
+        at += rdlane(iq, nk) - lead;
+      } else {
+        at += q - lead;
+      }
+      const uint32_t T = nk ? rdlane(incl, nk - 1u) : 0u;
+      S.item[i & 1][lane] = iq | ((incl - k) << 16);
+      if (lane == 0) {
+        S.nk[i & 1] = nk;
+        S.base[i & 1] = pos;
+        if (err != kOk || at >= len) S.last = i;
+      }
+      pos += T;
+      if (err == kOk && at < len) w = window_load(src, len, at, lane);
+    } else if (wave == 1 && i > 0) {
+      // write batch i-1
+      const uint32_t b = (i - 1) & 1;
+      const uint8_t *win = S.win[(i - 1) % 3];
+      const uint32_t nk = uni(S.nk[b]);
+      const uint32_t it = S.item[b][lane];
+      const uint32_t iq = it & 0xffffu, off = it >> 16;
+      const uint32_t kk0 = lane < nk ? (uint32_t)win[iq] : 0u;
+      const bool lit = kk0 > 127u;
+      const uint32_t kk = lane < nk ? (lit ? kk0 - 127u : kk0 + 3u) : 0u;
+      const uint32_t s0 = iq + 1u;
+      const uint32_t c = kk ? (uint32_t)win[s0] : 0u;
+      const uint32_t r0 = uni(S.base[b]) + off + o.mis;
+      const uint32_t hb = min((4u - (r0 & 3u)) & 3u, kk);
 #pragma unroll
-    for (uint32_t t = 0; t < 3u; t++)
-      if (tb + t < kk) S.ring[(r0 + tb + t) & kRingMask] = lit ? S.win[s + tb + t] : (uint8_t)c;
-    o.pos += T;
-    out_store(o, lane, o.pos, false);
-    if (err != kOk) break;
+      for (uint32_t t = 0; t < 3u; t++)
+        if (t < hb) S.ring[(r0 + t) & kRingMask] = lit ? win[s0 + t] : (uint8_t)c;
+      const uint32_t nd = (kk - hb) >> 2;
+      const uint32_t c4 = c * 0x01010101u;
+      uint32_t *ring32 = reinterpret_cast<uint32_t *>(S.ring);
+      const uint32_t *win32 = reinterpret_cast<const uint32_t *>(win);
+      for (uint32_t d = 0; d < nd; d++) {
+        const uint32_t u = s0 + hb + 4u * d;
+        uint32_t v = c4;
+        if (lit) {
+          const uint32_t a = u >> 2;
+          v = __builtin_amdgcn_alignbit(win32[a + 1u], win32[a], (u & 3u) * 8u);
+        }
+        ring32[((r0 + hb + 4u * d) & kRingMask) >> 2] = v;
+      }
+      const uint32_t tb = hb + 4u * nd;
+#pragma unroll
+      for (uint32_t t = 0; t < 3u; t++)
+        if (tb + t < kk) S.ring[(r0 + tb + t) & kRingMask] = lit ? win[s0 + tb + t] : (uint8_t)c;
+      // the batch's last byte: the largest item end
+      uint32_t e = lane < nk ? off + kk : 0u;
+#pragma unroll
+      for (uint32_t d = 32; d >= 1; d >>= 1) e = max(e, (uint32_t)__shfl_xor(e, (int)d, kWave));
+      o.pos = uni(S.base[b]) + uni(e);
+      out_store(o, lane, o.pos, false);
+    }
+    __syncthreads();
+    const uint32_t last = uni(S.last);
+    if (last != kNoLast && i >= last + 1u) break;
   }
-  out_store(o, lane, o.pos, true);
-  *produced = o.pos;
-  return err;
-}
-
-__global__ void __launch_bounds__(kWave) brle_chunks_kernel(const uint8_t *comp, uint64_t comp_bytes,
-                                                            const uint64_t *ptr, uint64_t n,
-                                                            uint64_t chunk_bytes, uint8_t *out,
-                                                            uint64_t *out_len, int32_t *status) {
-  __shared__ BrleLds S;
-  const uint32_t lane = lane_id();
-  const uint64_t i = blockIdx.x;
-  if (i >= n) return;
-  uint64_t len;
-  int32_t st;
-  chunk_bounds(ptr[i], ptr[i + 1], comp_bytes, len, st);
-  uint32_t produced = 0;
-  if (st == kOk)
-    st = brle_stream(comp + ptr[i], (uint32_t)len, out + i * chunk_bytes, (uint32_t)chunk_bytes, S,
-                     lane, &produced);
-  if (lane == 0) {
-    if (out_len) out_len[i] = produced;
-    if (status) status[i] = st;
+  if (wave == 1) out_store(o, lane, o.pos, true);
+  if (wave == 0 && lane == 0) {
+    if (out_len) out_len[ci] = pos;
+    if (status) status[ci] = err;
   }
 }
 
@@ -430,8 +464,8 @@ hipError_t launch_brle_chunks(const uint8_t *comp, uint64_t comp_bytes, const ui
                               uint64_t n, uint64_t chunk_bytes, uint8_t *out, uint64_t *out_len,
                               int32_t *status, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(bytecodec::brle_chunks_kernel, dim3((uint32_t)n), dim3(kWaveSize), 0, stream,
-                     comp, comp_bytes, ptr, n, chunk_bytes, out, out_len, status);
+  hipLaunchKernelGGL(bytecodec::brle_chunks2_kernel, dim3((uint32_t)n), dim3(2 * kWaveSize), 0,
+                     stream, comp, comp_bytes, ptr, n, chunk_bytes, out, out_len, status);
   return hipGetLastError();
 }
 
