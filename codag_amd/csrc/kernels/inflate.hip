@@ -105,9 +105,11 @@ constexpr uint32_t kRingMask = kRing - 1;
 constexpr uint32_t kBatchBytes = 512;
 constexpr uint32_t kFlushMin = 256;
 static_assert(kRing >= kBatchBytes + 258 + 64 + 258 + kFlushMin, "ring too small for the batch");
-// A symbol reads at most 15 + 5 + 15 + 13 bits: a batch that starts this far
-// from the stream end cannot read past it.
-constexpr uint32_t kCarefulBits = 64 * 48;
+// A symbol reads at most 15 + 5 + 15 + 13 bits: a full batch (64 symbols)
+// that starts this far from the stream end cannot read past it, and the
+// 96-bit field the window decoder loads from the dword of any of its symbols
+// lies inside the stream's dwords.
+constexpr uint32_t kCarefulBits = 64 * 48 + 96;
 constexpr uint32_t kGather = 8;  // far-match pieces in flight per memory round trip
 // The serial decode is latency-bound, so throughput grows with the waves
 // resident per SIMD: LDS (5760 B, twelve 512-byte allocation blocks per wave)
@@ -681,33 +683,30 @@ __device__ __forceinline__ void load_fixed(Lds &S) {
 }
 
 // The three stream dwords that hold the 64 bits from stream bit `at` (the
-// window decoder's per-lane input), as global loads clamped to the stream's
-// dwords; field_bits zeroes the dwords past the stream, as the bit reader
-// does, where the bits are used, so the loads stay in flight until then. A
-// full batch starts >= 384 bytes before the stream end, so nw > 0 where
-// these are called.
+// window decoder's per-lane input): one load, its first dword clamped to
+// nw - 3. The window decoder runs only for full batches, which start at
+// least kCarefulBits before the stream end, so at every position that starts
+// a symbol of the batch the three dwords are inside the stream and the clamp
+// changes only lanes no symbol of the batch starts at. The bits are taken
+// apart where they are used, so the load stays in flight until then.
 struct Field {
   uint32_t d0, d1, d2;
 };
 
 __device__ __forceinline__ Field load_field(const Bits &b, uint32_t at) {
   typedef __attribute__((address_space(1))) const uint32_t gdword;
-  const gdword *w = (const gdword *)b.w;
-  const uint32_t D = at >> 5, lastw = b.nw - 1;
+  const gdword *w = (const gdword *)b.w + min(at >> 5, b.nw - 3);
   Field f;
-  f.d0 = w[min(D, lastw)];
-  f.d1 = w[min(D + 1, lastw)];
-  f.d2 = w[min(D + 2, lastw)];
+  f.d0 = w[0];
+  f.d1 = w[1];
+  f.d2 = w[2];
   return f;
 }
 
-__device__ __forceinline__ uint64_t field_bits(const Bits &b, const Field &f, uint32_t at) {
-  const uint32_t D = at >> 5;
-  const uint32_t d0 = D < b.nw ? f.d0 : 0u;
-  const uint32_t d1 = D + 1 < b.nw ? f.d1 : 0u;
-  const uint32_t d2 = D + 2 < b.nw ? f.d2 : 0u;
-  const uint32_t xl = __builtin_amdgcn_alignbit(d1, d0, at & 31u);
-  const uint32_t xh = __builtin_amdgcn_alignbit(d2, d1, at & 31u);
+// v_alignbit_b32 shifts by its third operand's low five bits.
+__device__ __forceinline__ uint64_t field_bits(const Field &f, uint32_t at) {
+  const uint32_t xl = __builtin_amdgcn_alignbit(f.d1, f.d0, at);
+  const uint32_t xh = __builtin_amdgcn_alignbit(f.d2, f.d1, at);
   return ((uint64_t)xh << 32) | xl;
 }
 
@@ -719,19 +718,19 @@ __device__ __forceinline__ uint64_t field_bits(const Bits &b, const Field &f, ui
 // and an invalid code stop the chain (token length 128); the symbol loop
 // decodes that symbol. Batch limits are applied as the symbol loop applies
 // them: 64 tokens, or the match that brings the batch to kBatchBytes. Token k
-// of the batch goes to slot[k] in LDS and bit k of wm is set (the caller moves
-// those slots into tok once per batch); slots 64..79 take the writes of the
-// lanes that start no token. The next window's loads are issued as soon as
+// of the batch goes to slot[k] in LDS (the caller moves the slots into tok
+// once per batch); slots 64..79 take the writes of the lanes that start no
+// token. The next window's loads are issued as soon as
 // the chain has placed it, ahead of the batch-limit work.
 // Returns true when the batch is full, false at a symbol for the loop.
 __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
-                                              uint64_t &wm, uint32_t &nt, uint32_t &bb) {
+                                              uint32_t &nt, uint32_t &bb) {
   const uint32_t P0 = bits_pos(b);
   uint32_t P = P0;
   bool full;
   Field f = load_field(b, P + lane);
   for (;;) {
-    const uint64_t x = field_bits(b, f, P + lane);
+    const uint64_t x = field_bits(f, P + lane);
     const uint32_t xl = (uint32_t)x;
     const uint32_t e = S.lit[xl & kLitMask];
     const uint32_t L = e & 15, lx = (e >> 4) & 7;
@@ -753,7 +752,7 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     uint64_t M = 0;
     uint32_t j = 0;
     do {
-      M |= 1ull << j;
+      asm("s_bitset1_b64 %0, %1" : "+s"(M) : "s"(j));  // M |= 1 << j in one scalar operation
       j += rdlane(tl, j);
     } while (j < (uint32_t)kWave);
     const bool stop = j >= 128u;
@@ -761,11 +760,15 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     // the next window, unless this one stops or ends the batch
     f = load_field(b, P + adv + lane);
     // batch limits: the 64th token of the batch, or the match reaching
-    // kBatchBytes, ends it
-    const bool mk = ((M >> lane) & 1) && tl < 128u;
-    const uint32_t rank = lanes_below(__ballot(mk));
+    // kBatchBytes, ends it (nt < 64 and bb < kBatchBytes here). The token
+    // starts are kept as wave masks in scalar registers; inverse ballots
+    // hand them to the vector unit as conditions.
+    const uint64_t mkm = M & ~__ballot(tl >= 128u);  // a stop lane starts no token
+    const bool mk = __builtin_amdgcn_inverse_ballot_w64(mkm);
+    const uint32_t rank = lanes_below(mkm);
     const uint32_t incl = wave_scan_dpp(mk ? ml : 0u);
-    const uint64_t cm = __ballot(mk && (nt + rank + 1 >= (uint32_t)kWave || bb + incl >= kBatchBytes));
+    const uint64_t cm =
+        mkm & (__ballot(rank >= (uint32_t)kWave - 1 - nt) | __ballot(incl >= kBatchBytes - bb));
     uint32_t lastc = kWave - 1;
     if (cm) {
       lastc = (uint32_t)__builtin_ctzll(cm);
@@ -773,11 +776,9 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     } else {
       bb += rdlane(incl, kWave - 1);
     }
-    const bool keep = mk && lane <= lastc;
-    slot[keep ? nt + rank : kWave + (lane & 15)] = tk;
-    const uint32_t n = (uint32_t)__builtin_popcountll(__ballot(keep));
-    if (n) wm |= (~0ull >> (kWave - n)) << nt;
-    nt += n;
+    const uint64_t keepm = mkm & ((2ull << lastc) - 1);  // lastc = 63 keeps all
+    slot[__builtin_amdgcn_inverse_ballot_w64(keepm) ? nt + rank : kWave + (lane & 15)] = tk;
+    nt += (uint32_t)__builtin_popcountll(keepm);
     P += adv;
     if (cm || stop) {
       full = cm != 0;
@@ -789,7 +790,8 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
 }
 
 // Symbol loop: decodes up to `limit` (<= 64) symbols of the current block
-// into tok (lane j <- token j, nt tokens). Returns kOk when the batch is full
+// into tok (lane j <- token j, nt tokens); every token passes through its
+// LDS slot, written by lane 0 here and by decode_window. Returns kOk when the batch is full
 // (`limit` tokens, or the batch's matches reached kBatchBytes), kEob at
 // end-of-block, or the status of an invalid symbol. The tokens decoded before
 // an invalid symbol are valid and still have to be written. All stops leave
@@ -799,21 +801,20 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
 __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
                                                  uint32_t &tok, uint32_t &nt, uint32_t limit) {
   uint32_t bb = 0;
-  uint64_t wm = 0;  // batch tokens that are in the LDS slots
   int32_t st = kOk;
   for (;;) {
-    if (limit == (uint32_t)kWave && decode_window(b, S, slot, lane, wm, nt, bb)) break;
+    if (limit == (uint32_t)kWave && decode_window(b, S, slot, lane, nt, bb)) break;
     bits_refill(b);  // >= 33 bits
     uint32_t e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
     if (e & kLitFlag) {  // literal: the token is the table entry
       bits_drop(b, e & 15);
-      if (lane == nt) tok = e;
+      if (lane == 0) slot[nt] = e;
       if (++nt == limit) break;
       // >= 18 bits are left: enough for the next code without a refill
       e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
       if (e & kLitFlag) {
         bits_drop(b, e & 15);
-        if (lane == nt) tok = e;
+        if (lane == 0) slot[nt] = e;
         if (++nt == limit) break;
         continue;
       }
@@ -831,7 +832,7 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
     }
     bits_drop(b, L);
     if (e & kLitFlag) {
-      if (lane == nt) tok = e;
+      if (lane == 0) slot[nt] = e;
       if (++nt == limit) break;
       continue;
     }
@@ -863,13 +864,13 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
     const uint32_t dx = (d >> 4) & 15;
     const uint32_t dist = (((d >> 8) & 3) << dx) + 1 + ((uint32_t)b.buf & ((1u << dx) - 1));
     bits_drop(b, dx);
-    if (lane == nt) tok = len | (dist << 16);
+    if (lane == 0) slot[nt] = len | (dist << 16);
     bb += len;
     if (++nt == limit || bb >= kBatchBytes) break;
   }
-  if (wm) {
-    const uint32_t got = slot[lane];
-    if ((wm >> lane) & 1) tok = got;
+  if (nt) {
+    wave_sync();
+    tok = slot[lane];
   }
   return st;
 }
@@ -890,7 +891,7 @@ __device__ __forceinline__ int32_t decode_block(Bits &b, Sink &out, const Lds &S
     uint32_t limit = (uint32_t)kWave;
     if (left < kCarefulBits) {
       const uint32_t safe = (left * 1365u) >> 16;
-      limit = safe ? safe : 1u;
+      limit = safe == 0 ? 1u : (safe < (uint32_t)kWave ? safe : (uint32_t)kWave - 1);
     }
     uint32_t tok = 0, nt = 0;
     const int32_t sa = decode_tokens(b, S, slot, lane, tok, nt, limit);
