@@ -769,19 +769,23 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     const uint32_t incl = wave_scan_dpp(mk ? ml : 0u);
     const uint64_t cm =
         mkm & (__ballot(rank >= (uint32_t)kWave - 1 - nt) | __ballot(incl >= kBatchBytes - bb));
-    uint32_t lastc = kWave - 1;
-    if (cm) {
-      lastc = (uint32_t)__builtin_ctzll(cm);
-      adv = lastc + rdlane(tl, lastc);
-    } else {
-      bb += rdlane(incl, kWave - 1);
+    // The two exits and the common path each store their own slots, so no
+    // flag has to live across a join.
+    if (cm) {  // the batch ends at token lastc
+      const uint32_t lastc = (uint32_t)__builtin_ctzll(cm);
+      const uint64_t keepm = mkm & ((2ull << lastc) - 1);
+      slot[__builtin_amdgcn_inverse_ballot_w64(keepm) ? nt + rank : kWave + (lane & 15)] = tk;
+      nt += (uint32_t)__builtin_popcountll(keepm);
+      P += lastc + rdlane(tl, lastc);
+      full = true;
+      break;
     }
-    const uint64_t keepm = mkm & ((2ull << lastc) - 1);  // lastc = 63 keeps all
-    slot[__builtin_amdgcn_inverse_ballot_w64(keepm) ? nt + rank : kWave + (lane & 15)] = tk;
-    nt += (uint32_t)__builtin_popcountll(keepm);
+    bb += rdlane(incl, kWave - 1);
+    slot[mk ? nt + rank : kWave + (lane & 15)] = tk;
+    nt += (uint32_t)__builtin_popcountll(mkm);
     P += adv;
-    if (cm || stop) {
-      full = cm != 0;
+    if (stop) {
+      full = false;
       break;
     }
   }
