@@ -122,8 +122,8 @@ void default_tune(codag_ctx *c) {
   // autotuner can pick per column
   c->tune[CODAG_RLEV1] = {512, 2048, 0, 4};
   c->tune[CODAG_RLEV2] = {128, 4096, 0, 2};
-  // 28 waves per CU: the inflate kernel's LDS and register budget allow seven per SIMD
-  c->tune[CODAG_DEFLATE] = {128, 0, 12, 1};  // two-wave batch workgroups
+  // two-wave batch workgroups, 12 per CU (the one-wave shape allows up to 32)
+  c->tune[CODAG_DEFLATE] = {128, 0, 12, 1};
   c->tune[0] = {0, 0, 0, 0};
 }
 
@@ -290,10 +290,10 @@ codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_
   if (n_chunks > 0x7fffffffull) return fail(CODAG_ERR_INVALID, "inflate_chunks: too many chunks");
   DeviceGuard g(ctx->device);
   // Two-wave workgroups (a decoding and a writing wave per chunk) shorten
-  // each chunk's decode but halve the chunks in flight: they pay while every
-  // chunk is resident at once (up to 14 two-wave workgroups per CU), not
-  // when chunks queue for slots.
-  bool two_wave = n_chunks <= (uint64_t)ctx->cu_count * 14;
+  // each chunk's decode but hold fewer chunks in flight than one-wave
+  // workgroups (32 per CU): measured on 128 KiB chunks, they pay up to about
+  // 12 chunks per CU.
+  bool two_wave = n_chunks <= (uint64_t)ctx->cu_count * 12;
   if (ctx->chunks_two_wave) two_wave = ctx->chunks_two_wave == 1;
   HIP_TRY(launch_inflate_chunks(d_comp, comp_bytes, d_offsets, n_chunks, chunk_bytes, d_out,
                                 d_out_len, d_status, flags, pick(ctx, stream), two_wave));

@@ -27,7 +27,7 @@
 //    Decoding never depends on output bytes, so it runs without waiting on
 //    memory beyond its own table reads and stream loads.
 //  * The ring is written to HBM with 16-byte stores after each batch that
-//    leaves at least kFlushMin unwritten bytes; Adler-32 partial sums (sum b,
+//    leaves at least kFlush unwritten bytes; Adler-32 partial sums (sum b,
 //    sum pos*b) are accumulated on the way, so the trailer is verified without
 //    a second pass over the output.
 //
@@ -93,31 +93,38 @@ template <int NSYM> struct Huff {
   uint16_t sym[NSYM];
 };
 
-// Output ring and batch limits. A batch holds at most 64 tokens and stops once
-// its matches reach kBatchBytes, so it produces T < kBatchBytes + 258 + 64
-// bytes. At a batch start fewer than kFlushMin bytes are unwritten to HBM.
-// Then (a) the batch never overwrites a ring byte that is not yet in HBM and
-// (b) a match whose source lies before the ring's oldest intact byte reads
-// only bytes that are already in HBM, when kRing >= T + 258 + kFlushMin.
-// Sized so a wave needs under 6 KiB of LDS.
-constexpr uint32_t kRing = 2048;
-constexpr uint32_t kRingMask = kRing - 1;
-constexpr uint32_t kBatchBytes = 512;
-constexpr uint32_t kFlushMin = 256;
-static_assert(kRing >= kBatchBytes + 258 + 64 + 258 + kFlushMin, "ring too small for the batch");
+// Output ring and batch limits. A ring of R bytes takes batches of at most 64
+// tokens that stop once their matches reach kBatch = R / 4 bytes, so a batch
+// produces T < kBatch + 258 + 64 bytes, and is flushed to HBM once kFlush =
+// R / 8 bytes are pending, so fewer than kFlush bytes are unwritten at a
+// batch start. Then (a) a batch never overwrites a ring byte that is not yet
+// in HBM and (b) a match whose source lies before the ring's oldest intact
+// byte reads only bytes that are already in HBM, when R >= T + 258 + kFlush.
+// One-wave kernels use a 1 KiB ring, which leaves a wave under 5 KiB of LDS
+// (32 waves per CU); the writing wave of a two-wave group has a 2 KiB ring
+// (fewer far gathers, fewer batches).
+template <uint32_t R> struct RingGeo {
+  static constexpr uint32_t kRing = R, kMask = R - 1, kBatch = R / 4, kFlush = R / 8;
+  static_assert((R & (R - 1)) == 0 && R >= kBatch + 258 + 64 + 258 + kFlush, "ring too small");
+};
+constexpr uint32_t kRingOne = 1024;
+constexpr uint32_t kRingPair = 2048;
 // A symbol reads at most 15 + 5 + 15 + 13 bits: a full batch (64 symbols)
 // that starts this far from the stream end cannot read past it, and the
 // 96-bit field the window decoder loads from the dword of any of its symbols
 // lies inside the stream's dwords.
 constexpr uint32_t kCarefulBits = 64 * 48 + 96;
 constexpr uint32_t kGather = 8;  // far-match pieces in flight per memory round trip
-// The serial decode is latency-bound, so throughput grows with the waves
-// resident per SIMD: LDS (5760 B, twelve 512-byte allocation blocks per wave)
-// allows 26 per CU, and the register budget is held to seven per SIMD.
-constexpr int kWavesPerSimd = 7;
+// A wave's serial decode is latency-bound, so throughput grows with the waves
+// resident per SIMD: a one-wave kernel's LDS (4736 B, ten 512-byte allocation
+// blocks) allows 32 per CU, and its register budget is held to eight per SIMD
+// (64 VGPRs), so a 1 GiB container of 128 KiB chunks is decoded in one round.
+// Two-wave groups hold seven per SIMD.
+constexpr int kWavesOne = 8;
+constexpr int kWavesPair = 7;
 
 struct Lds {
-  alignas(16) uint8_t ring[kRing];
+  alignas(16) uint8_t ring[kRingOne];
   uint16_t lit[1u << kLitBits];
   uint16_t dist[1u << kDistBits];
   Huff<288> hl;
@@ -125,10 +132,9 @@ struct Lds {
   alignas(4) uint8_t lens[320];
 };
 static_assert(sizeof(Lds::lens) >= 4 * (kWave + 16), "lens doubles as the token slots");
-// ring and lit are adjacent: their 4 KiB hold the CRC-32 slice tables while
-// no block is being decoded (gzip header and trailer)
-static_assert(offsetof(Lds, lit) == kRing && sizeof(Lds::ring) + sizeof(Lds::lit) == 4096,
-              "CRC tables need ring + lit");
+// The first 4 KiB (ring, lit, dist and the start of hl) hold the CRC-32 slice
+// tables while no block is being decoded (gzip header and trailer)
+static_assert(sizeof(Lds) >= 4096, "CRC tables need 4 KiB");
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t j) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
@@ -229,12 +235,12 @@ __device__ __forceinline__ void bits_seek(Bits &b, uint64_t lead_plus_at) {
 
 // ------------------------------------------------------------- output side
 
-// The most recent kRing output bytes live in an LDS ring. Byte x of the
-// stream sits at ring[(x + mis) & (kRing-1)], where mis = dst & 15, so a
+// The most recent R output bytes live in an LDS ring. Byte x of the
+// stream sits at ring[(x + mis) & (R - 1)], where mis = dst & 15, so a
 // 16-byte aligned output address is also 16-byte aligned in the ring.
 // Positions are 32-bit (outputs below 2^31 bytes) so the wave-uniform state
 // stays in scalar registers.
-struct Out {
+template <uint32_t R> struct Out : RingGeo<R> {
   uint8_t *dst;      // global output
   uint8_t *ring;     // LDS ring
   uint32_t cap;
@@ -247,18 +253,18 @@ struct Out {
   uint32_t s1, s2;
 };
 
-__device__ __forceinline__ uint32_t ridx(const Out &o, uint32_t x) {
-  return (x + o.mis) & kRingMask;
+template <class O> __device__ __forceinline__ uint32_t ridx(const O &o, uint32_t x) {
+  return (x + o.mis) & O::kMask;
 }
 
-__device__ __forceinline__ void adler_add(Out &o, uint32_t at, uint32_t b) {
+template <class O> __device__ __forceinline__ void adler_add(O &o, uint32_t at, uint32_t b) {
   o.s1 += b;
   o.s2 += (at % 65521u) * b;
 }
 
 // Stores ring bytes [flushed, end) to HBM; `last` also writes a partial
 // 16-byte tail.
-__device__ __forceinline__ void out_store(Out &o, uint32_t lane, uint32_t end, bool last) {
+template <class O> __device__ __forceinline__ void out_store(O &o, uint32_t lane, uint32_t end, bool last) {
   uint32_t x = o.flushed;
   // head bytes up to a 16-byte aligned output address
   uint32_t head = (16 - ((o.mis + x) & 15)) & 15;
@@ -315,7 +321,8 @@ __device__ __forceinline__ void out_store(Out &o, uint32_t lane, uint32_t end, b
 // Copies far matches (lanes in fm) from HBM into the ring. The first 64 bytes
 // of kGather matches are loaded before any is written, so a group costs one
 // memory round trip; the rest of the (rare) longer far matches follows.
-__device__ __forceinline__ void gather_far(Out &o, uint32_t lane, uint64_t fm, uint32_t x,
+template <class O>
+__device__ __forceinline__ void gather_far(O &o, uint32_t lane, uint64_t fm, uint32_t x,
                                            uint32_t dist, uint32_t mlen) {
   uint64_t g = fm;
   while (g) {
@@ -368,26 +375,36 @@ __constant__ Recip16 kRecip16 = Recip16();
 //   match: length in bits 8:0 (bit 15 clear), distance in bits 31:16.
 // Errors are reported for the first offending token, in stream order:
 // output past cap (overflow) or a distance beyond the bytes produced so far.
-__device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t tok, uint32_t nt) {
+// The tokens before it are still written, so the bytes written before an
+// error do not depend on where batches end (ring sizes differ by kernel).
+template <class O>
+__device__ __forceinline__ int32_t out_tokens(O &o, uint32_t lane, uint32_t tok, uint32_t nt) {
   if (nt == 0) return kOk;
-  const bool act = lane < nt;
-  const bool lit = act && (tok & kLitFlag);
-  const bool mat = act && !(tok & kLitFlag);
+  const bool isl = (tok & kLitFlag) != 0;
   const uint32_t mlen = tok & 0x1ff;
   const uint32_t dist = tok >> 16;
-  const uint32_t tl = lit ? 1u : (mat ? mlen : 0u);
+  const uint32_t tl = lane < nt ? (isl ? 1u : mlen) : 0u;
   const uint32_t incl = wave_scan_dpp(tl);
   const uint32_t x = o.pos + incl - tl;  // output position of this token
   uint32_t err = 0;
-  if (act && x + tl > o.cap) err = kOverflow;
-  else if (mat && dist > x) err = kCorrupt;  // zlib: "invalid distance too far back"
+  if (lane < nt && x + tl > o.cap) err = kOverflow;
+  else if (lane < nt && !isl && dist > x) err = kCorrupt;  // zlib: "invalid distance too far back"
   const uint64_t em = __ballot(err != 0);
-  if (em) return (int32_t)rdlane(err, (uint32_t)__builtin_ctzll(em));
+  int32_t st = kOk;
+  if (em) {
+    const uint32_t k = (uint32_t)__builtin_ctzll(em);
+    st = (int32_t)rdlane(err, k);
+    nt = k;
+    if (nt == 0) return st;
+  }
+  const bool act = lane < nt;
+  const bool lit = act && isl;
+  const bool mat = act && !isl;
   const uint32_t pend = o.pos + rdlane(incl, nt - 1);
 
   if (lit) o.ring[ridx(o, x)] = (uint8_t)(tok >> 4);
-  // a source byte y is intact in the ring iff y >= pend - kRing
-  const bool far = mat && (x - dist) + kRing < pend;
+  // a source byte y is intact in the ring iff y >= pend - R
+  const bool far = mat && (x - dist) + O::kRing < pend;
   const uint64_t fm = __ballot(far);
   if (fm) gather_far(o, lane, fm, x, dist, mlen);
   // A near match whose source ends before the batch's first byte reads only
@@ -439,13 +456,14 @@ __device__ __forceinline__ int32_t out_tokens(Out &o, uint32_t lane, uint32_t to
     }
   }
   o.pos = pend;
-  if (pend - o.flushed >= kFlushMin) out_store(o, lane, pend, false);
-  return kOk;
+  if (pend - o.flushed >= O::kFlush) out_store(o, lane, pend, false);
+  return st;
 }
 
 // Copies ln stored-block bytes from src + at through the ring (RFC 1951
 // 3.2.4), 256 bytes per step.
-__device__ __forceinline__ int32_t store_copy(Out &o, const uint8_t *src, uint64_t at, uint32_t ln,
+template <class O>
+__device__ __forceinline__ int32_t store_copy(O &o, const uint8_t *src, uint64_t at, uint32_t ln,
                                               uint32_t lane) {
   if (o.pos + ln > o.cap) return kOverflow;
   for (uint32_t done = 0; done < ln;) {
@@ -454,14 +472,14 @@ __device__ __forceinline__ int32_t store_copy(Out &o, const uint8_t *src, uint64
     for (uint32_t i = lane; i < step; i += kWave) o.ring[ridx(o, o.pos + i)] = src[at + done + i];
     o.pos += step;
     done += step;
-    if (o.pos - o.flushed >= kFlushMin) out_store(o, lane, o.pos, false);
+    if (o.pos - o.flushed >= O::kFlush) out_store(o, lane, o.pos, false);
   }
   return kOk;
 }
 
 // The last partial flush, then the Adler-32 of the bytes written (a wave
 // reduction of the per-lane partial sums, each already < 65521).
-__device__ __forceinline__ uint32_t out_finish(Out &o, uint32_t lane) {
+template <class O> __device__ __forceinline__ uint32_t out_finish(O &o, uint32_t lane) {
   out_store(o, lane, o.pos, true);
   uint32_t s1r = o.s1, s2r = o.s2;
 #pragma unroll
@@ -717,12 +735,13 @@ __device__ __forceinline__ uint64_t field_bits(const Field &f, uint32_t at) {
 // real token starts. A token whose code is not in the LUT, an end-of-block
 // and an invalid code stop the chain (token length 128); the symbol loop
 // decodes that symbol. Batch limits are applied as the symbol loop applies
-// them: 64 tokens, or the match that brings the batch to kBatchBytes. Token k
+// them: 64 tokens, or the match that brings the batch to kBatch bytes. Token k
 // of the batch goes to slot[k] in LDS (the caller moves the slots into tok
 // once per batch); slots 64..79 take the writes of the lanes that start no
 // token. The next window's loads are issued as soon as
 // the chain has placed it, ahead of the batch-limit work.
 // Returns true when the batch is full, false at a symbol for the loop.
+template <uint32_t kBatch>
 __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
                                               uint32_t &nt, uint32_t &bb) {
   const uint32_t P0 = bits_pos(b);
@@ -760,7 +779,7 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     // the next window, unless this one stops or ends the batch
     f = load_field(b, P + adv + lane);
     // batch limits: the 64th token of the batch, or the match reaching
-    // kBatchBytes, ends it (nt < 64 and bb < kBatchBytes here). The token
+    // kBatch bytes, ends it (nt < 64 and bb < kBatch here). The token
     // starts are kept as wave masks in scalar registers; inverse ballots
     // hand them to the vector unit as conditions.
     const uint64_t mkm = M & ~__ballot(tl >= 128u);  // a stop lane starts no token
@@ -768,7 +787,7 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     const uint32_t rank = lanes_below(mkm);
     const uint32_t incl = wave_scan_dpp(mk ? ml : 0u);
     const uint64_t cm =
-        mkm & (__ballot(rank >= (uint32_t)kWave - 1 - nt) | __ballot(incl >= kBatchBytes - bb));
+        mkm & (__ballot(rank >= (uint32_t)kWave - 1 - nt) | __ballot(incl >= kBatch - bb));
     // The two exits and the common path each store their own slots, so no
     // flag has to live across a join.
     if (cm) {  // the batch ends at token lastc
@@ -795,19 +814,20 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
 
 // Symbol loop: decodes up to `limit` (<= 64) symbols of the current block
 // into tok (lane j <- token j, nt tokens); every token passes through its
-// LDS slot, written by lane 0 here and by decode_window. Returns kOk when the batch is full
-// (`limit` tokens, or the batch's matches reached kBatchBytes), kEob at
-// end-of-block, or the status of an invalid symbol. The tokens decoded before
+// LDS slot, written by lane 0 here and by decode_window. Returns kOk when
+// the batch is full (`limit` tokens, or the batch's matches reached kBatch
+// bytes), kEob at end-of-block, or the status of an invalid symbol. The tokens decoded before
 // an invalid symbol are valid and still have to be written. All stops leave
 // through one exit so the loop's uniform state stays in scalar registers.
 // A full batch (limit 64) is decoded by decode_window, and the loop takes
 // only the symbols that stop it.
+template <uint32_t kBatch>
 __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
                                                  uint32_t &tok, uint32_t &nt, uint32_t limit) {
   uint32_t bb = 0;
   int32_t st = kOk;
   for (;;) {
-    if (limit == (uint32_t)kWave && decode_window(b, S, slot, lane, nt, bb)) break;
+    if (limit == (uint32_t)kWave && decode_window<kBatch>(b, S, slot, lane, nt, bb)) break;
     bits_refill(b);  // >= 33 bits
     uint32_t e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
     if (e & kLitFlag) {  // literal: the token is the table entry
@@ -870,7 +890,7 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
     bits_drop(b, dx);
     if (lane == 0) slot[nt] = len | (dist << 16);
     bb += len;
-    if (++nt == limit || bb >= kBatchBytes) break;
+    if (++nt == limit || bb >= kBatch) break;
   }
   if (nt) {
     wave_sync();
@@ -898,7 +918,7 @@ __device__ __forceinline__ int32_t decode_block(Bits &b, Sink &out, const Lds &S
       limit = safe == 0 ? 1u : (safe < (uint32_t)kWave ? safe : (uint32_t)kWave - 1);
     }
     uint32_t tok = 0, nt = 0;
-    const int32_t sa = decode_tokens(b, S, slot, lane, tok, nt, limit);
+    const int32_t sa = decode_tokens<Sink::kBatch>(b, S, slot, lane, tok, nt, limit);
     if (bits_past_end(b)) return kTrunc;
     const int32_t sb = out.tokens(lane, tok, nt);
     if (sb != kOk) return sb;
@@ -913,9 +933,11 @@ struct Result {
   uint32_t adler;
 };
 
-// Output of a stream decoded by one wave: the wave writes its own batches.
-struct SinkOne {
-  Out o;
+// Output of a stream decoded by one wave: the wave writes its own batches
+// through an LDS ring of R bytes.
+template <uint32_t R> struct SinkOne {
+  static constexpr uint32_t kBatch = RingGeo<R>::kBatch;
+  Out<R> o;
   __device__ __forceinline__ void init(uint8_t *dst, uint8_t *ring, uint64_t cap) {
     o.dst = dst;
     o.ring = ring;
@@ -1206,12 +1228,12 @@ __device__ __forceinline__ Result inflate_stream_t(const uint8_t *src, uint64_t 
 
 __device__ __forceinline__ Result inflate_stream(const uint8_t *src, uint64_t len, uint8_t *dst,
                                                  uint64_t cap, uint32_t flags, Lds &S) {
-  SinkOne out;
+  SinkOne<kRingOne> out;
   out.init(dst, S.ring, cap > 0x7fffffffull ? 0x7fffffffull : cap);
   return inflate_stream_t(src, len, dst, cap, flags, S, out);
 }
 
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesOne)))
 inflate_chunks_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                       const uint64_t *__restrict__ off, uint64_t n, uint64_t chunk_bytes, uint8_t *__restrict__ out,
                       uint64_t *__restrict__ out_len, int32_t *__restrict__ status, uint32_t flags) {
@@ -1230,7 +1252,7 @@ inflate_chunks_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
   }
 }
 
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesOne)))
 inflate_batch_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
                      codag_stream_result *__restrict__ results, uint32_t flags,
                      uint32_t *__restrict__ counter) {
@@ -1301,6 +1323,7 @@ __device__ __forceinline__ void lds_release(uint32_t *p, uint32_t v) {
 
 // The decoding wave's side: batches go into the queue.
 struct SinkQueue {
+  static constexpr uint32_t kBatch = RingGeo<kRingPair>::kBatch;  // the writing wave's ring
   Queue *q;
   uint32_t head;
   bool ended, stuck;
@@ -1359,7 +1382,7 @@ struct SinkQueue {
 __device__ __forceinline__ void inflate_write(Queue &q, uint8_t *ring, uint8_t *dst, uint64_t cap,
                                               const uint8_t *src) {
   const uint32_t lane = lane_id();
-  SinkOne w;
+  SinkOne<kRingPair> w;
   w.init(dst, ring, cap > 0x7fffffffull ? 0x7fffffffull : cap);
   int32_t wst = kOk;
   for (uint32_t t = 0;; t++) {
@@ -1404,11 +1427,11 @@ __device__ __forceinline__ void inflate_write(Queue &q, uint8_t *ring, uint8_t *
 
 struct Lds2 {
   Lds dec;                       // the decoding wave's tables, slots and CRC space
-  alignas(16) uint8_t ring[kRing];  // the writing wave's output ring
+  alignas(16) uint8_t ring[kRingPair];  // the writing wave's output ring
   Queue q;
 };
 
-__global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
+__global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(kWavesPair)))
 inflate_batch2_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
                       codag_stream_result *__restrict__ results, uint32_t flags,
                       uint32_t *__restrict__ counter) {
@@ -1457,7 +1480,7 @@ inflate_batch2_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
 }
 
 // The chunk container decoded by two-wave workgroups, one chunk each.
-__global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd)))
+__global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(kWavesPair)))
 inflate_chunks2_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
                        const uint64_t *__restrict__ off, uint64_t n, uint64_t chunk_bytes,
                        uint8_t *__restrict__ out, uint64_t *__restrict__ out_len,
