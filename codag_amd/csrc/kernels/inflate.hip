@@ -753,17 +753,22 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     const uint32_t xl = (uint32_t)x;
     const uint32_t e = S.lit[xl & kLitMask];
     const uint32_t L = e & 15, lx = (e >> 4) & 7;
-    const uint64_t y = x >> (L + lx);
-    const uint32_t d = S.dist[(uint32_t)y & kDistMask];
+    // a length code's extra bits, its distance code and that code's extra
+    // bits lie within the 32 bits from bit L + lx (< 32) on
+    const uint32_t y = __builtin_amdgcn_alignbit((uint32_t)(x >> 32), xl, L + lx);
+    const uint32_t d = S.dist[y & kDistMask];
     const uint32_t dl = d & 15, dx = (d >> 4) & 15;
-    const uint32_t mlen = ((e >> 7) & 255) + 3 + ((uint32_t)(x >> L) & ((1u << lx) - 1));
-    const uint32_t dist = (((d >> 8) & 3) << dx) + 1 + ((uint32_t)(y >> dl) & ((1u << dx) - 1));
-    const bool lit = (e & kLitFlag) != 0;
+    const uint32_t mlen = ((e >> 7) & 255) + 3 + __builtin_amdgcn_ubfe(xl, L, lx);
+    const uint32_t dist = (((d >> 8) & 3) << dx) + 1 + __builtin_amdgcn_ubfe(y, dl, dx);
+    const uint64_t litm = __ballot(e >= kLitFlag);
+    const bool lit = __builtin_amdgcn_inverse_ballot_w64(litm);
     // a length code stops the chain if it or its distance code is not in the
     // LUT, if it is end-of-block or invalid (extra field 6, 7), or if the
     // distance code is 30 or 31
     const uint32_t bad = (((L - 1u) | (dl - 1u)) >> 31) | ((lx + 2u) >> 3) | ((d & kDistBad) >> 13);
-    const uint32_t tl = lit ? L : (bad ? 128u : L + lx + dl + dx);
+    uint32_t tlm = bad ? 128u : L + lx + dl + dx;
+    asm("" : "+v"(tlm));  // every lane computes it: no branch around the literal lanes
+    const uint32_t tl = lit ? L : tlm;
     const uint32_t tk = lit ? e : (mlen | (dist << 16));
     const uint32_t ml = lit ? 0u : mlen;
     // chain: token starts from window bit 0 while they lie in the window; a
@@ -771,8 +776,11 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     uint64_t M = 0;
     uint32_t j = 0;
     do {
-      asm("s_bitset1_b64 %0, %1" : "+s"(M) : "s"(j));  // M |= 1 << j in one scalar operation
-      j += rdlane(tl, j);
+      // M |= 1 << j, then tl of lane j; the scalar operation also gives the
+      // wait state a v_readlane needs after the vector write of tl
+      uint32_t t;
+      asm("s_bitset1_b64 %0, %2\n\tv_readlane_b32 %1, %3, %2" : "+s"(M), "=s"(t) : "s"(j), "v"(tl));
+      j += t;
     } while (j < (uint32_t)kWave);
     const bool stop = j >= 128u;
     uint32_t adv = j & 127u;  // bits used by the tokens before the stop / window end
