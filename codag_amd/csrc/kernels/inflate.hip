@@ -93,19 +93,19 @@ template <int NSYM> struct Huff {
   uint16_t sym[NSYM];
 };
 
-// Output ring and batch limits. A ring of R bytes takes batches of at most 64
-// tokens that stop once their matches reach kBatch = R / 4 bytes, so a batch
-// produces T < kBatch + 258 + 64 bytes, and is flushed to HBM once kFlush =
-// R / 8 bytes are pending, so fewer than kFlush bytes are unwritten at a
-// batch start. Then (a) a batch never overwrites a ring byte that is not yet
-// in HBM and (b) a match whose source lies before the ring's oldest intact
-// byte reads only bytes that are already in HBM, when R >= T + 258 + kFlush.
-// One-wave kernels use a 1 KiB ring, which leaves a wave under 5 KiB of LDS
-// (32 waves per CU); the writing wave of a two-wave group has a 2 KiB ring
-// (fewer far gathers, fewer batches).
+// Output ring and piece limits. The writer materialises a batch (at most 64
+// tokens) in pieces: a piece ends at the first token at which its bytes
+// reach kBatch = R / 4, so it produces T < kBatch + 258 bytes, and the ring
+// is flushed to HBM once kFlush = R / 8 bytes are pending, so fewer than
+// kFlush bytes are unwritten at a piece start. Then (a) a piece never
+// overwrites a ring byte that is not yet in HBM and (b) a match whose source
+// lies before the ring's oldest intact byte reads only bytes that are
+// already in HBM, when R >= T + 258 + kFlush. One-wave kernels use a 1 KiB
+// ring, which leaves a wave under 5 KiB of LDS (32 waves per CU); the
+// writing wave of a two-wave group has a 2 KiB ring (fewer far gathers).
 template <uint32_t R> struct RingGeo {
   static constexpr uint32_t kRing = R, kMask = R - 1, kBatch = R / 4, kFlush = R / 8;
-  static_assert((R & (R - 1)) == 0 && R >= kBatch + 258 + 64 + 258 + kFlush, "ring too small");
+  static_assert((R & (R - 1)) == 0 && R >= kBatch + 258 + 258 + kFlush, "ring too small");
 };
 constexpr uint32_t kRingOne = 1024;
 constexpr uint32_t kRingPair = 2048;
@@ -370,46 +370,42 @@ struct Recip16 {
 };
 __constant__ Recip16 kRecip16 = Recip16();
 
-// Materialises a batch of nt tokens (lane j holds token j):
-//   literal: its LUT entry (bit 15 set, byte in bits 11:4);
-//   match: length in bits 8:0 (bit 15 clear), distance in bits 31:16.
+// Materialises the tokens of lanes [a, b) of a batch as one piece: rel is
+// the token's inclusive byte count from the piece start, tl its byte count.
 // Errors are reported for the first offending token, in stream order:
 // output past cap (overflow) or a distance beyond the bytes produced so far.
 // The tokens before it are still written, so the bytes written before an
-// error do not depend on where batches end (ring sizes differ by kernel).
+// error do not depend on where batches and pieces end.
 template <class O>
-__device__ __forceinline__ int32_t out_tokens(O &o, uint32_t lane, uint32_t tok, uint32_t nt) {
-  if (nt == 0) return kOk;
-  const bool isl = (tok & kLitFlag) != 0;
-  const uint32_t mlen = tok & 0x1ff;
-  const uint32_t dist = tok >> 16;
-  const uint32_t tl = lane < nt ? (isl ? 1u : mlen) : 0u;
-  const uint32_t incl = wave_scan_dpp(tl);
-  const uint32_t x = o.pos + incl - tl;  // output position of this token
+__device__ __forceinline__ int32_t out_piece(O &o, uint32_t lane, uint32_t tok, bool isl, uint32_t mlen,
+                                             uint32_t dist, uint32_t tl, uint32_t rel, uint32_t a,
+                                             uint32_t b) {
+  const uint32_t x = o.pos + rel - tl;  // output position of this token
+  const bool inp = lane >= a && lane < b;
   uint32_t err = 0;
-  if (lane < nt && x + tl > o.cap) err = kOverflow;
-  else if (lane < nt && !isl && dist > x) err = kCorrupt;  // zlib: "invalid distance too far back"
+  if (inp && x + tl > o.cap) err = kOverflow;
+  else if (inp && !isl && dist > x) err = kCorrupt;  // zlib: "invalid distance too far back"
   const uint64_t em = __ballot(err != 0);
   int32_t st = kOk;
   if (em) {
     const uint32_t k = (uint32_t)__builtin_ctzll(em);
     st = (int32_t)rdlane(err, k);
-    nt = k;
-    if (nt == 0) return st;
+    b = k;
+    if (b == a) return st;
   }
-  const bool act = lane < nt;
+  const bool act = lane >= a && lane < b;
   const bool lit = act && isl;
   const bool mat = act && !isl;
-  const uint32_t pend = o.pos + rdlane(incl, nt - 1);
+  const uint32_t pend = o.pos + rdlane(rel, b - 1);
 
   if (lit) o.ring[ridx(o, x)] = (uint8_t)(tok >> 4);
   // a source byte y is intact in the ring iff y >= pend - R
   const bool far = mat && (x - dist) + O::kRing < pend;
   const uint64_t fm = __ballot(far);
   if (fm) gather_far(o, lane, fm, x, dist, mlen);
-  // A near match whose source ends before the batch's first byte reads only
-  // bytes that were in the ring before the batch, and writes bytes that no
-  // earlier token of the batch reads: such matches of up to 16 bytes are
+  // A near match whose source ends before the piece's first byte reads only
+  // bytes that were in the ring before the piece, and writes bytes that no
+  // earlier token of the piece reads: such matches of up to 16 bytes are
   // copied together, each lane its own, 4 bytes per step (bytes past the
   // match's end repeat its last byte's copy). The other near matches follow
   // in token order.
@@ -458,6 +454,32 @@ __device__ __forceinline__ int32_t out_tokens(O &o, uint32_t lane, uint32_t tok,
   o.pos = pend;
   if (pend - o.flushed >= O::kFlush) out_store(o, lane, pend, false);
   return st;
+}
+
+// Materialises a batch of nt tokens (lane j holds token j):
+//   literal: its LUT entry (bit 15 set, byte in bits 11:4);
+//   match: length in bits 8:0 (bit 15 clear), distance in bits 31:16.
+// A batch of up to 64 tokens can produce up to 64 x 258 bytes, more than the
+// ring holds, so it is written in pieces: a piece ends at the first token at
+// which its bytes reach O::kBatch, so a piece produces T < kBatch + 258
+// bytes. The decoder only limits batches to 64 tokens.
+template <class O>
+__device__ __forceinline__ int32_t out_tokens(O &o, uint32_t lane, uint32_t tok, uint32_t nt) {
+  if (nt == 0) return kOk;
+  const bool isl = (tok & kLitFlag) != 0;
+  const uint32_t mlen = tok & 0x1ff;
+  const uint32_t dist = tok >> 16;
+  const uint32_t tl = lane < nt ? (isl ? 1u : mlen) : 0u;
+  const uint32_t incl = wave_scan_dpp(tl);
+  uint32_t a = 0, base = 0;  // the piece's first lane; bytes of the tokens before it
+  for (;;) {
+    const uint64_t over = __ballot(lane >= a && lane < nt && incl - base >= O::kBatch);
+    const uint32_t b = over ? (uint32_t)__builtin_ctzll(over) + 1 : nt;
+    const int32_t st = out_piece(o, lane, tok, isl, mlen, dist, tl, incl - base, a, b);
+    if (st != kOk || b == nt) return st;
+    base = rdlane(incl, b - 1);
+    a = b;
+  }
 }
 
 // Copies ln stored-block bytes from src + at through the ring (RFC 1951
@@ -734,16 +756,14 @@ __device__ __forceinline__ uint64_t field_bits(const Field &f, uint32_t at) {
 // the symbol loop; a scalar chain through the token lengths then keeps the
 // real token starts. A token whose code is not in the LUT, an end-of-block
 // and an invalid code stop the chain (token length 128); the symbol loop
-// decodes that symbol. Batch limits are applied as the symbol loop applies
-// them: 64 tokens, or the match that brings the batch to kBatch bytes. Token k
-// of the batch goes to slot[k] in LDS (the caller moves the slots into tok
-// once per batch); slots 64..79 take the writes of the lanes that start no
-// token. The next window's loads are issued as soon as
-// the chain has placed it, ahead of the batch-limit work.
+// decodes that symbol. A batch ends at its 64th token (the writer cuts it
+// into pieces that fit its ring). Token k of the batch goes to slot[k] in
+// LDS (the caller moves the slots into tok once per batch); slots 64..79
+// take the writes of the lanes that start no token. The next window's loads
+// are issued as soon as the chain has placed it, ahead of the slot work.
 // Returns true when the batch is full, false at a symbol for the loop.
-template <uint32_t kBatch>
 __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
-                                              uint32_t &nt, uint32_t &bb) {
+                                              uint32_t &nt) {
   const uint32_t P0 = bits_pos(b);
   uint32_t P = P0;
   bool full;
@@ -770,7 +790,6 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     asm("" : "+v"(tlm));  // every lane computes it: no branch around the literal lanes
     const uint32_t tl = lit ? L : tlm;
     const uint32_t tk = lit ? e : (mlen | (dist << 16));
-    const uint32_t ml = lit ? 0u : mlen;
     // chain: token starts from window bit 0 while they lie in the window; a
     // stop lane (length 128) ends it at j >= 128
     uint64_t M = 0;
@@ -786,19 +805,15 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     uint32_t adv = j & 127u;  // bits used by the tokens before the stop / window end
     // the next window, unless this one stops or ends the batch
     f = load_field(b, P + adv + lane);
-    // batch limits: the 64th token of the batch, or the match reaching
-    // kBatch bytes, ends it (nt < 64 and bb < kBatch here). The token
-    // starts are kept as wave masks in scalar registers; inverse ballots
-    // hand them to the vector unit as conditions.
+    // The batch's 64th token ends it (nt < 64 here). The token starts are
+    // kept as wave masks in scalar registers; inverse ballots hand them to
+    // the vector unit as conditions. The two exits and the common path each
+    // store their own slots, so no flag has to live across a join.
     const uint64_t mkm = M & ~__ballot(tl >= 128u);  // a stop lane starts no token
-    const bool mk = __builtin_amdgcn_inverse_ballot_w64(mkm);
     const uint32_t rank = lanes_below(mkm);
-    const uint32_t incl = wave_scan_dpp(mk ? ml : 0u);
-    const uint64_t cm =
-        mkm & (__ballot(rank >= (uint32_t)kWave - 1 - nt) | __ballot(incl >= kBatch - bb));
-    // The two exits and the common path each store their own slots, so no
-    // flag has to live across a join.
-    if (cm) {  // the batch ends at token lastc
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(mkm);
+    if (nt + cnt >= (uint32_t)kWave) {  // the batch ends at token lastc
+      const uint64_t cm = mkm & __ballot(rank >= (uint32_t)kWave - 1 - nt);
       const uint32_t lastc = (uint32_t)__builtin_ctzll(cm);
       const uint64_t keepm = mkm & ((2ull << lastc) - 1);
       slot[__builtin_amdgcn_inverse_ballot_w64(keepm) ? nt + rank : kWave + (lane & 15)] = tk;
@@ -807,9 +822,8 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
       full = true;
       break;
     }
-    bb += rdlane(incl, kWave - 1);
-    slot[mk ? nt + rank : kWave + (lane & 15)] = tk;
-    nt += (uint32_t)__builtin_popcountll(mkm);
+    slot[__builtin_amdgcn_inverse_ballot_w64(mkm) ? nt + rank : kWave + (lane & 15)] = tk;
+    nt += cnt;
     P += adv;
     if (stop) {
       full = false;
@@ -823,19 +837,17 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
 // Symbol loop: decodes up to `limit` (<= 64) symbols of the current block
 // into tok (lane j <- token j, nt tokens); every token passes through its
 // LDS slot, written by lane 0 here and by decode_window. Returns kOk when
-// the batch is full (`limit` tokens, or the batch's matches reached kBatch
-// bytes), kEob at end-of-block, or the status of an invalid symbol. The tokens decoded before
+// the batch is full (`limit` tokens), kEob at end-of-block, or the status
+// of an invalid symbol. The tokens decoded before
 // an invalid symbol are valid and still have to be written. All stops leave
 // through one exit so the loop's uniform state stays in scalar registers.
 // A full batch (limit 64) is decoded by decode_window, and the loop takes
 // only the symbols that stop it.
-template <uint32_t kBatch>
 __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
                                                  uint32_t &tok, uint32_t &nt, uint32_t limit) {
-  uint32_t bb = 0;
   int32_t st = kOk;
   for (;;) {
-    if (limit == (uint32_t)kWave && decode_window<kBatch>(b, S, slot, lane, nt, bb)) break;
+    if (limit == (uint32_t)kWave && decode_window(b, S, slot, lane, nt)) break;
     bits_refill(b);  // >= 33 bits
     uint32_t e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
     if (e & kLitFlag) {  // literal: the token is the table entry
@@ -897,8 +909,7 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
     const uint32_t dist = (((d >> 8) & 3) << dx) + 1 + ((uint32_t)b.buf & ((1u << dx) - 1));
     bits_drop(b, dx);
     if (lane == 0) slot[nt] = len | (dist << 16);
-    bb += len;
-    if (++nt == limit || bb >= kBatch) break;
+    if (++nt == limit) break;
   }
   if (nt) {
     wave_sync();
@@ -926,7 +937,7 @@ __device__ __forceinline__ int32_t decode_block(Bits &b, Sink &out, const Lds &S
       limit = safe == 0 ? 1u : (safe < (uint32_t)kWave ? safe : (uint32_t)kWave - 1);
     }
     uint32_t tok = 0, nt = 0;
-    const int32_t sa = decode_tokens<Sink::kBatch>(b, S, slot, lane, tok, nt, limit);
+    const int32_t sa = decode_tokens(b, S, slot, lane, tok, nt, limit);
     if (bits_past_end(b)) return kTrunc;
     const int32_t sb = out.tokens(lane, tok, nt);
     if (sb != kOk) return sb;
@@ -944,7 +955,6 @@ struct Result {
 // Output of a stream decoded by one wave: the wave writes its own batches
 // through an LDS ring of R bytes.
 template <uint32_t R> struct SinkOne {
-  static constexpr uint32_t kBatch = RingGeo<R>::kBatch;
   Out<R> o;
   __device__ __forceinline__ void init(uint8_t *dst, uint8_t *ring, uint64_t cap) {
     o.dst = dst;
@@ -1331,7 +1341,6 @@ __device__ __forceinline__ void lds_release(uint32_t *p, uint32_t v) {
 
 // The decoding wave's side: batches go into the queue.
 struct SinkQueue {
-  static constexpr uint32_t kBatch = RingGeo<kRingPair>::kBatch;  // the writing wave's ring
   Queue *q;
   uint32_t head;
   bool ended, stuck;
