@@ -480,9 +480,25 @@ def bench_batch(ctx, args):
     d_descs = ctx.upload(np.frombuffer(descs, np.uint8))
     rsz = 16
     d_res = ctx.alloc(n * rsz)
+    tuned = None
     if args.c4_shape:
         tpg, per = (int(x) for x in args.c4_shape.split(","))
         ctx.tune_set(ca.DEFLATE, ca.TuneParams(tpg, 0, per, 1))
+    elif not args.no_autotune:
+        # the autotuner times its launch shapes (one or two waves per stream,
+        # workgroups per CU) on the largest streams, which set the batch's
+        # finish time; it keeps the fastest shape whose output matches
+        k = min(n, 512)
+        ch = int(cap[0])
+        d_soff = ctx.upload(soff[: k + 1].astype(np.uint64))
+        d_tout = ctx.alloc(k * ch)
+        tbest, tms = ctx.tune_run(ca.DEFLATE, d_src, int(soff[k]), d_soff, None, k, 1, ch, d_tout,
+                                  reps=2)
+        del d_tout, d_soff
+        tuned = {"sample": f"the {k} largest streams", "threads_per_group": tbest.threads_per_group,
+                 "groups_per_cu": tbest.groups_per_cu, "sample_ms": round(tms, 3)}
+        log(f"[bench] c4: autotuner on {tuned['sample']}: {tbest.threads_per_group} threads x "
+            f"{tbest.groups_per_cu} workgroups per CU ({tms:.1f} ms)")
     best = ctx.tune_get(ca.DEFLATE)
     kname = "inflate_batch2_kernel" if best.threads_per_group == 128 else "inflate_batch_kernel"
     log(f"[bench] c4: persistent launch, {best.threads_per_group} threads x "
@@ -564,6 +580,7 @@ def bench_batch(ctx, args):
         "decompressed_bytes": total, "compressed_bytes": int(slen.sum()),
         "blocks_fixed_dynamic_stored": [int(x) for x in mix], "crc32": crc,
         "groups_per_cu": int(best.groups_per_cu), "threads_per_group": int(best.threads_per_group),
+        "autotune": tuned,
         "wall_s": round(wall, 3),
         "per_file_launch_gbs": round(pf_gbs, 3),
         "per_file_sample": f"{len(samp)} streams, one inflate launch each",

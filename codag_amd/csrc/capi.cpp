@@ -848,7 +848,7 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
   const codag_tune_params saved = ctx->tune[codec];
   const uint64_t out_bytes = n_chunks * chunk_bytes;
   std::vector<uint8_t> ref, cur;
-  float bestt = 1e30f;
+  float bestt = 1e30f, saved_ms = 0;
   codag_tune_params bestp = saved;
   DevBuf d_st, d_descs, d_res;
   HIP_TRY(d_st.alloc(n_chunks * 8 + 8));
@@ -901,6 +901,13 @@ codag_status codag_tune_run(codag_ctx *ctx, codag_codec codec, const uint8_t *d_
       bestt = ms;
       bestp = cands[ci];
     }
+    if (same && std::memcmp(&cands[ci], &saved, sizeof saved) == 0) saved_ms = ms;
+  }
+  // the shape in use stays unless a candidate beats it by more than 2%
+  // (run-to-run noise must not flip the choice)
+  if (saved_ms > 0 && saved_ms <= bestt * 1.02f) {
+    bestp = saved;
+    bestt = saved_ms;
   }
   ctx->tune[codec] = bestp;
   if (best) *best = bestp;

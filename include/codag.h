@@ -408,7 +408,9 @@ codag_status codag_tune_set(codag_ctx *ctx, codag_codec codec, const codag_tune_
 /* Time each candidate on the given device-resident sample (same arguments as
  * the matching decode call; for DEFLATE d_blk_off are the n+1 offsets and
  * d_col_len is ignored) and keep the fastest whose output equals the first
- * candidate's. best_ms (nullable) gets its time. DEFLATE candidates differ in
+ * candidate's; the shape in use stays if it is within 2% of that one (a tie
+ * in run-to-run noise does not flip the choice). The kept shape becomes the
+ * context's shape. best_ms (nullable) gets its time. DEFLATE candidates differ in
  * the batch launch's workgroup (one or two waves per stream) and workgroups
  * per CU; the launch has min(CUs x groups_per_cu, n_chunks) workgroups, so
  * the sample needs more than CUs x 28 streams for all of them to differ. */
