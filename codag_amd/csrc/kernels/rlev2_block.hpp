@@ -88,8 +88,9 @@ __device__ __forceinline__ typename Col<W>::T be_bytes(const uint8_t *s, uint32_
 // the header walk, so its two DELTA varints are at most 10 bytes each.
 //
 // LDS reads are issued unconditionally (lanes past a block's count read its
-// element 0) and two 64-element groups at a time, so reads that do not depend
-// on each other share one wait.
+// element 0) and two 64-element groups at a time while more than 64 elements
+// are left, so reads that do not depend on each other share one wait; the
+// last group of 64 or fewer is read alone.
 template <uint32_t W>
 __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint32_t o, uint8_t *dst,
                                            uint32_t lane, uint32_t *bad) {
@@ -113,10 +114,15 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
     const uint32_t b0 = (p + 2) * 8;
     for (uint32_t c = 0; c < len8; c += 2 * kWave) {
       const uint32_t i0 = c + lane, i1 = c + kWave + lane;
-      const T v0 = field<W>(sbuf, b0 + __umul24(i0 < len8 ? i0 : 0u, fbw), fbw) & mask;
-      const T v1 = field<W>(sbuf, b0 + __umul24(i1 < len8 ? i1 : 0u, fbw), fbw) & mask;
-      if (i0 < len8) store_w<W>(dst, o + i0, v0);
-      if (i1 < len8) store_w<W>(dst, o + i1, v1);
+      if (len8 - c > (uint32_t)kWave) {
+        const T v0 = field<W>(sbuf, b0 + __umul24(i0, fbw), fbw) & mask;
+        const T v1 = field<W>(sbuf, b0 + __umul24(i1 < len8 ? i1 : 0u, fbw), fbw) & mask;
+        store_w<W>(dst, o + i0, v0);
+        if (i1 < len8) store_w<W>(dst, o + i1, v1);
+      } else {
+        const T v0 = field<W>(sbuf, b0 + __umul24(i0 < len8 ? i0 : 0u, fbw), fbw) & mask;
+        if (i0 < len8) store_w<W>(dst, o + i0, v0);
+      }
     }
   } else if (kind == 3) {  // DELTA (decoder_new.h:1045-1093)
     // The varints of base and delta base start at byte p + 2 and end at the
@@ -169,10 +175,15 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
       };
       for (uint32_t c = 0; c < nd8; c += 2 * kWave) {  // uniform loop: all lanes scan
         const uint32_t i0 = c + lane, i1 = c + kWave + lane;
-        const T r0 = field<W>(sbuf, b0 + __umul24(i0 < nd8 ? i0 : 0u, fbw), fbw) & mask;
-        const T r1 = field<W>(sbuf, b0 + __umul24(i1 < nd8 ? i1 : 0u, fbw), fbw) & mask;
-        step(i0, i0 < nd8 ? r0 : (T)0);
-        if (c + kWave < nd8) step(i1, i1 < nd8 ? r1 : (T)0);
+        if (nd8 - c > (uint32_t)kWave) {
+          const T r0 = field<W>(sbuf, b0 + __umul24(i0, fbw), fbw) & mask;
+          const T r1 = field<W>(sbuf, b0 + __umul24(i1 < nd8 ? i1 : 0u, fbw), fbw) & mask;
+          step(i0, r0);
+          step(i1, i1 < nd8 ? r1 : (T)0);
+        } else {
+          const T r0 = field<W>(sbuf, b0 + __umul24(i0 < nd8 ? i0 : 0u, fbw), fbw) & mask;
+          step(i0, i0 < nd8 ? r0 : (T)0);
+        }
       }
     }
   } else {  // PATCHED_BASE (decoder_new.h:1094-1183)
@@ -198,20 +209,35 @@ __device__ __forceinline__ void expand_big(const uint8_t *sbuf, uint32_t p, uint
     const uint32_t gsum = wave_scan_dpp(g);
     const T add = (fbw >= nbits) ? (T)0 : (T)(((ent & pmask) << fbw) & (uint64_t)mask);
     if (lane < pll && gsum >= len8) *bad = 1;
+    // the OR-term of patch entry j (held by lane j < pll)
+    auto patch = [&](uint32_t j) -> T {
+      T aj = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)add, (int)j);
+      if (W > 4)
+        aj |= (T)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)add >> 32), (int)j) << 32);
+      return aj;
+    };
     for (uint32_t c = 0; c < len8; c += 2 * kWave) {
       const uint32_t i0 = c + lane, i1 = c + kWave + lane;
-      T v0 = field<W>(sbuf, vb0 + __umul24(i0 < len8 ? i0 : 0u, fbw), fbw) & mask;
-      T v1 = field<W>(sbuf, vb0 + __umul24(i1 < len8 ? i1 : 0u, fbw), fbw) & mask;
-      for (uint32_t j = 0; j < pll; j++) {  // patch entries read from their lanes
-        const uint32_t gj = (uint32_t)__builtin_amdgcn_readlane((int)gsum, (int)j);
-        T aj = (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)add, (int)j);
-        if (W > 4)
-          aj |= (T)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)add >> 32), (int)j) << 32);
-        if (gj == i0) v0 |= aj;
-        if (gj == i1) v1 |= aj;
+      if (len8 - c > (uint32_t)kWave) {
+        T v0 = field<W>(sbuf, vb0 + __umul24(i0, fbw), fbw) & mask;
+        T v1 = field<W>(sbuf, vb0 + __umul24(i1 < len8 ? i1 : 0u, fbw), fbw) & mask;
+        for (uint32_t j = 0; j < pll; j++) {  // patch entries read from their lanes
+          const uint32_t gj = (uint32_t)__builtin_amdgcn_readlane((int)gsum, (int)j);
+          const T aj = patch(j);
+          if (gj == i0) v0 |= aj;
+          if (gj == i1) v1 |= aj;
+        }
+        store_w<W>(dst, o + i0, (T)(v0 + base) & mask);
+        if (i1 < len8) store_w<W>(dst, o + i1, (T)(v1 + base) & mask);
+      } else {
+        T v0 = field<W>(sbuf, vb0 + __umul24(i0 < len8 ? i0 : 0u, fbw), fbw) & mask;
+        for (uint32_t j = 0; j < pll; j++) {
+          const uint32_t gj = (uint32_t)__builtin_amdgcn_readlane((int)gsum, (int)j);
+          const T aj = patch(j);
+          if (gj == i0) v0 |= aj;
+        }
+        if (i0 < len8) store_w<W>(dst, o + i0, (T)(v0 + base) & mask);
       }
-      if (i0 < len8) store_w<W>(dst, o + i0, (T)(v0 + base) & mask);
-      if (i1 < len8) store_w<W>(dst, o + i1, (T)(v1 + base) & mask);
     }
   }
 }
