@@ -18,18 +18,22 @@
 //         the chain (codes longer than the LUT, end-of-block, invalid codes)
 //         and the batches near the stream end. Token j of the batch is parked
 //         in lane j of a VGPR;
-//      2. the whole wave materialises the batch: a wave prefix sum gives each
-//         token its output offset, literals go to an LDS ring in one store,
-//         matches whose source has left the ring are gathered from HBM (eight
-//         per memory round trip), short matches whose source precedes the
-//         batch are copied together (one lane each), and the remaining
-//         matches are copied ring -> ring in token order, 64 bytes per step.
+//      2. the writer (this wave, or the second wave of a two-wave group)
+//         materialises the batch in ring-sized pieces: a wave prefix sum
+//         gives each token its output offset, literals go to an LDS ring in
+//         one store, matches whose source has left the ring are gathered from
+//         HBM (eight per memory round trip), short matches whose source
+//         precedes the piece are copied together (one lane each), and the
+//         remaining matches are copied ring -> ring in token order, 64 bytes
+//         per step.
 //    Decoding never depends on output bytes, so it runs without waiting on
 //    memory beyond its own table reads and stream loads.
-//  * The ring is written to HBM with 16-byte stores after each batch that
+//  * The ring is written to HBM with 16-byte stores after each piece that
 //    leaves at least kFlush unwritten bytes; Adler-32 partial sums (sum b,
 //    sum pos*b) are accumulated on the way, so the trailer is verified without
 //    a second pass over the output.
+//  * Occupancy: a one-wave kernel uses a 1 KiB ring and 4736 B of LDS (32
+//    waves per CU); a two-wave group gives its writer a 2 KiB ring.
 //
 // Reference counterpart: include/zlib/zlib.h:807-866 (full_warp_shared_lut),
 // 531-707 (dynamic header / table build), 712-805 (symbol loop), 188-347
