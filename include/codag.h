@@ -122,9 +122,8 @@ enum {
  * d_offsets: n_chunks+1 byte offsets into d_comp (the blk_offset file minus
  * its leading length word, zlib.h:1603); stream i is [off[i], off[i+1]) and
  * inflates into d_out + i*chunk_bytes (at most chunk_bytes).
- * d_out_len (nullable): bytes produced per chunk. On an error these are the
- * bytes of every symbol decoded before it (the same for every launch
- * shape). The zlib header is checked as zlib does (the reference skips it
+ * d_out_len (nullable): bytes produced per chunk (on an error, the bytes
+ * stored before it; the count does not depend on the launch shape). The zlib header is checked as zlib does (the reference skips it
  * unchecked, zlib.h:819). */
 codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
                                   const uint64_t *d_offsets, uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *d_out,
@@ -147,7 +146,7 @@ typedef struct codag_stream_desc {
 } codag_stream_desc;
 
 typedef struct codag_stream_result {
-  uint64_t bytes_written;  /* on an error: the bytes of every symbol decoded before it */
+  uint64_t bytes_written;  /* on an error: bytes stored before it (any launch shape) */
   int32_t status;    /* CODAG_CHUNK_* (cuDF codes with CODAG_INFLATE_CUDF_STATUS) */
   uint32_t adler32;  /* Adler-32 of the bytes written (CRC-32 with CODAG_INFLATE_GZIP) */
 } codag_stream_result;
