@@ -123,8 +123,9 @@ enum {
  * its leading length word, zlib.h:1603); stream i is [off[i], off[i+1]) and
  * inflates into d_out + i*chunk_bytes (at most chunk_bytes).
  * d_out_len (nullable): bytes produced per chunk (on an error, the bytes
- * stored before it; the count does not depend on the launch shape). The zlib header is checked as zlib does (the reference skips it
- * unchecked, zlib.h:819). */
+ * stored before it; the count does not depend on the launch shape). The
+ * zlib header is checked as zlib does (the reference skips it unchecked,
+ * zlib.h:819). */
 codag_status codag_inflate_chunks(codag_ctx *ctx, const uint8_t *d_comp, uint64_t comp_bytes,
                                   const uint64_t *d_offsets, uint64_t n_chunks, uint64_t chunk_bytes, uint8_t *d_out,
                                   uint64_t *d_out_len, int32_t *d_chunk_status, uint32_t flags,
