@@ -612,25 +612,23 @@ __device__ __noinline__ int32_t build_table(const uint8_t *lens, uint32_t n, uin
   return kOk;
 }
 
-// Canonical decode of codes longer than the LUT (or invalid ones), bit by bit
-// (the count/first walk of RFC 1951 canonical codes). Returns the symbol or
-// -1; *len gets the code length.
+// Canonical decode of codes longer than the LUT (or invalid ones), RFC 1951
+// 3.2.2: the code of length L is the first L stream bits read MSB-first, and
+// it is valid when it lies in [first[L], first[L] + count[L]). Lane L (1..15)
+// tests length L, so the tables are read once instead of once per length; the
+// shortest length that hits is the code (canonical codes are prefix-free).
+// Called by the whole wave. Returns the symbol or -1; *len gets the length.
 template <int NSYM>
 __device__ __forceinline__ int32_t slow_decode(const Huff<NSYM> &h, uint64_t buf, uint32_t *len) {
-  uint32_t code = 0, first = 0, index = 0;
-  for (uint32_t L = 1; L < 16; L++) {
-    code |= (uint32_t)(buf & 1);
-    buf >>= 1;
-    const uint32_t count = uni(h.count[L]);
-    if (code - first < count) {
-      *len = L;
-      return (int32_t)uni(h.sym[index + code - first]);
-    }
-    index += count;
-    first = (first + count) << 1;
-    code <<= 1;
-  }
-  return -1;
+  const uint32_t lane = lane_id();
+  const uint32_t L = lane & 15u;
+  const uint32_t c = h.count[L], f = h.first[L], ix = h.index[L];
+  const uint32_t code = __builtin_bitreverse32((uint32_t)buf) >> ((32u - L) & 31u);  // L = 0: unused
+  const uint64_t m = __ballot(lane >= 1u && lane < 16u && code - f < c);
+  if (m == 0) return -1;
+  const uint32_t hit = (uint32_t)__builtin_ctzll(m);
+  *len = hit;
+  return (int32_t)uni(h.sym[rdlane(ix + code - f, hit)]);
 }
 
 // ------------------------------------------------------------- stream decode
