@@ -794,15 +794,29 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     const uint32_t tk = lit ? e : (mlen | (dist << 16));
     // chain: token starts from window bit 0 while they lie in the window; a
     // stop lane (length 128) ends it at j >= 128
+    // M |= 1 << j, then tl of lane j, while j < 64. Written out eight links
+    // at a time with a fall-through test per link: a taken branch per link
+    // (the loop's back edge) costs a lone wave about as much as the link
+    // itself. The scalar bitset also gives the wait state a v_readlane needs
+    // after the vector write of tl.
     uint64_t M = 0;
     uint32_t j = 0;
-    do {
-      // M |= 1 << j, then tl of lane j; the scalar operation also gives the
-      // wait state a v_readlane needs after the vector write of tl
+    {
       uint32_t t;
-      asm("s_bitset1_b64 %0, %2\n\tv_readlane_b32 %1, %3, %2" : "+s"(M), "=s"(t) : "s"(j), "v"(tl));
-      j += t;
-    } while (j < (uint32_t)kWave);
+      asm("1:\n"
+          ".rept 8\n"
+          "s_bitset1_b64 %[M], %[j]\n"
+          "v_readlane_b32 %[t], %[tl], %[j]\n"
+          "s_add_u32 %[j], %[j], %[t]\n"
+          "s_cmp_gt_u32 %[j], 63\n"
+          "s_cbranch_scc1 2f\n"
+          ".endr\n"
+          "s_branch 1b\n"
+          "2:"
+          : [j] "+s"(j), [M] "+s"(M), [t] "=&s"(t)
+          : [tl] "v"(tl)
+          : "scc");
+    }
     const bool stop = j >= 128u;
     uint32_t adv = j & 127u;  // bits used by the tokens before the stop / window end
     // the next window, unless this one stops or ends the batch
