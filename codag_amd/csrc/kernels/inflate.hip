@@ -797,10 +797,12 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
     // M |= 1 << j, then tl of lane j, while j < 64. Written out eight links
     // at a time with a fall-through test per link: a taken branch per link
     // (the loop's back edge) costs a lone wave about as much as the link
-    // itself. The scalar bitset also gives the wait state a v_readlane needs
-    // after the vector write of tl.
+    // itself. j is kept as j - 64, so the add's carry is the test, and the
+    // bitset and the lane select use only its low six bits. The scalar
+    // bitset also gives the wait state a v_readlane needs after the vector
+    // write of tl.
     uint64_t M = 0;
-    uint32_t j = 0;
+    uint32_t j = 0u - (uint32_t)kWave;  // j - 64 (mod 2^32): the add carries once j >= 64
     {
       uint32_t t;
       asm("1:\n"
@@ -808,7 +810,6 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
           "s_bitset1_b64 %[M], %[j]\n"
           "v_readlane_b32 %[t], %[tl], %[j]\n"
           "s_add_u32 %[j], %[j], %[t]\n"
-          "s_cmp_gt_u32 %[j], 63\n"
           "s_cbranch_scc1 2f\n"
           ".endr\n"
           "s_branch 1b\n"
@@ -817,6 +818,7 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
           : [tl] "v"(tl)
           : "scc");
     }
+    j += (uint32_t)kWave;
     const bool stop = j >= 128u;
     uint32_t adv = j & 127u;  // bits used by the tokens before the stop / window end
     // the next window, unless this one stops or ends the batch
