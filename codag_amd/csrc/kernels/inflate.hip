@@ -6,8 +6,9 @@
 //    -- and refilled into the bit buffer with v_readlane, so a refill costs no
 //    memory round trip; the next window is prefetched with one coalesced load.
 //  * Huffman tables: per-wave LDS lookup tables built per block by all 64
-//    lanes (10-bit literal/length table, 8-bit distance table, canonical
-//    count/symbol arrays for the rare longer codes). Table entries carry the
+//    lanes (literal/length table of 10 bits in the one-wave kernels and 11
+//    in the decoding wave of a two-wave group, 8-bit distance table,
+//    canonical count/symbol arrays for the rare longer codes). Table entries carry the
 //    RFC 1951 length/distance base and extra-bit count directly; a literal's
 //    entry has bit 15 set so the symbol loop tests it with one scalar compare.
 //  * Two phases per batch of up to 64 symbols:
@@ -51,10 +52,13 @@ namespace inflate {
 
 using namespace codag::dev;
 
-constexpr uint32_t kLitBits = 10;
+// literal/length LUT bits: 10 for the one-wave kernels (32 waves fit a CU),
+// 11 for the decoding wave of a two-wave group (fewer codes leave the
+// window decoder for the symbol loop)
+constexpr uint32_t kLitBitsOne = 10;
+constexpr uint32_t kLitBitsPair = 11;
 constexpr uint32_t kDistBits = 8;
 constexpr uint32_t kCodeBits = 7;
-constexpr uint32_t kLitMask = (1u << kLitBits) - 1;
 constexpr uint32_t kDistMask = (1u << kDistBits) - 1;
 
 enum : int32_t { kOk = 0, kTrunc = 1, kOverflow = 2, kCorrupt = 3 };
@@ -127,14 +131,17 @@ constexpr uint32_t kGather = 8;  // far-match pieces in flight per memory round 
 constexpr int kWavesOne = 8;
 constexpr int kWavesPair = 7;
 
-struct Lds {
+template <uint32_t LB> struct LdsT {
+  static constexpr uint32_t kLitBits = LB, kLitMask = (1u << LB) - 1;
   alignas(16) uint8_t ring[kRingOne];
-  uint16_t lit[1u << kLitBits];
+  uint16_t lit[1u << LB];
   uint16_t dist[1u << kDistBits];
   Huff<288> hl;
   Huff<32> hd;
   alignas(4) uint8_t lens[320];
 };
+using Lds = LdsT<kLitBitsOne>;
+using LdsPair = LdsT<kLitBitsPair>;
 static_assert(sizeof(Lds::lens) >= 4 * (kWave + 16), "lens doubles as the token slots");
 // The first 4 KiB (ring, lit, dist and the start of hl) hold the CRC-32 slice
 // tables while no block is being decoded (gzip header and trailer)
@@ -642,7 +649,8 @@ __device__ __forceinline__ int32_t status_code(int32_t st, uint32_t flags) {
   return st == kOverflow ? 1 : (st == kTrunc ? 2 : -1);
 }
 
-__device__ __forceinline__ int32_t read_dynamic(Bits &b, Lds &S, uint32_t *nlen_out) {
+template <class LD>
+__device__ __forceinline__ int32_t read_dynamic(Bits &b, LD &S, uint32_t *nlen_out) {
   const uint32_t lane = lane_id();
   bits_refill(b);
   const uint32_t nlen = (uint32_t)(b.buf & 31) + 257;
@@ -706,19 +714,19 @@ __device__ __forceinline__ int32_t read_dynamic(Bits &b, Lds &S, uint32_t *nlen_
   if (bits_past_end(b)) return kTrunc;
   wave_sync();
   if (uni(S.lens[256]) == 0) return kCorrupt;  // zlib: "missing end-of-block"
-  st = (int32_t)uni((uint32_t)build_table(S.lens, nlen, kTabLit, S.hl, S.lit, kLitBits));
+  st = (int32_t)uni((uint32_t)build_table(S.lens, nlen, kTabLit, S.hl, S.lit, LD::kLitBits));
   if (st != kOk) return st;
   st = (int32_t)uni((uint32_t)build_table(S.lens + nlen, ndist, kTabDist, S.hd, S.dist, kDistBits));
   *nlen_out = nlen;
   return st;
 }
 
-__device__ __forceinline__ void load_fixed(Lds &S) {
+template <class LD> __device__ __forceinline__ void load_fixed(LD &S) {
   const uint32_t lane = lane_id();
   for (uint32_t s = lane; s < 288; s += kWave)
     S.lens[s] = (uint8_t)(s < 144 ? 8 : (s < 256 ? 9 : (s < 280 ? 7 : 8)));
   wave_sync();
-  build_table(S.lens, 288, kTabLit, S.hl, S.lit, kLitBits);
+  build_table(S.lens, 288, kTabLit, S.hl, S.lit, LD::kLitBits);
   for (uint32_t s = lane; s < 32; s += kWave) S.lens[s] = 5;
   wave_sync();
   build_table(S.lens, 32, kTabDist, S.hd, S.dist, kDistBits);
@@ -764,7 +772,8 @@ __device__ __forceinline__ uint64_t field_bits(const Field &f, uint32_t at) {
 // take the writes of the lanes that start no token. The next window's loads
 // are issued as soon as the chain has placed it, ahead of the slot work.
 // Returns true when the batch is full, false at a symbol for the loop.
-__device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
+template <class LD>
+__device__ __forceinline__ bool decode_window(Bits &b, const LD &S, uint32_t *slot, uint32_t lane,
                                               uint32_t &nt) {
   const uint32_t P0 = bits_pos(b);
   uint32_t P = P0;
@@ -773,7 +782,7 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
   for (;;) {
     const uint64_t x = field_bits(f, P + lane);
     const uint32_t xl = (uint32_t)x;
-    const uint32_t e = S.lit[xl & kLitMask];
+    const uint32_t e = S.lit[xl & LD::kLitMask];
     const uint32_t L = e & 15, lx = (e >> 4) & 7;
     // a length code's extra bits, its distance code and that code's extra
     // bits lie within the 32 bits from bit L + lx (< 32) on
@@ -861,19 +870,20 @@ __device__ __forceinline__ bool decode_window(Bits &b, const Lds &S, uint32_t *s
 // through one exit so the loop's uniform state stays in scalar registers.
 // A full batch (limit 64) is decoded by decode_window, and the loop takes
 // only the symbols that stop it.
-__device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t *slot, uint32_t lane,
+template <class LD>
+__device__ __forceinline__ int32_t decode_tokens(Bits &b, const LD &S, uint32_t *slot, uint32_t lane,
                                                  uint32_t &tok, uint32_t &nt, uint32_t limit) {
   int32_t st = kOk;
   for (;;) {
     if (limit == (uint32_t)kWave && decode_window(b, S, slot, lane, nt)) break;
     bits_refill(b);  // >= 33 bits
-    uint32_t e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
+    uint32_t e = uni(S.lit[(uint32_t)b.buf & LD::kLitMask]);
     if (e & kLitFlag) {  // literal: the token is the table entry
       bits_drop(b, e & 15);
       if (lane == 0) slot[nt] = e;
       if (++nt == limit) break;
       // >= 18 bits are left: enough for the next code without a refill
-      e = uni(S.lit[(uint32_t)b.buf & kLitMask]);
+      e = uni(S.lit[(uint32_t)b.buf & LD::kLitMask]);
       if (e & kLitFlag) {
         bits_drop(b, e & 15);
         if (lane == 0) slot[nt] = e;
@@ -939,8 +949,8 @@ __device__ __forceinline__ int32_t decode_tokens(Bits &b, const Lds &S, uint32_t
 // Decodes symbols until end-of-block. Returns a status. The sink takes each
 // batch (SinkOne writes it on this wave, SinkQueue hands it to the writing
 // wave) and returns the first output error seen so far.
-template <class Sink>
-__device__ __forceinline__ int32_t decode_block(Bits &b, Sink &out, const Lds &S, uint32_t *slot,
+template <class Sink, class LD>
+__device__ __forceinline__ int32_t decode_block(Bits &b, Sink &out, const LD &S, uint32_t *slot,
                                                 uint32_t lane) {
   for (;;) {
     // A symbol reads at most 48 bits, so a batch of floor(left / 48) symbols
@@ -1140,9 +1150,9 @@ __device__ __forceinline__ int64_t gzip_header(const uint8_t *src, uint64_t len,
 
 // Decodes one zlib / raw / gzip stream; `out` takes the decoded batches
 // (SinkOne: this wave writes them; SinkQueue: a second wave does).
-template <class Sink>
+template <class Sink, class LD>
 __device__ __forceinline__ Result inflate_stream_t(const uint8_t *src, uint64_t len, uint8_t *dst,
-                                                   uint64_t cap, uint32_t flags, Lds &S, Sink &out) {
+                                                   uint64_t cap, uint32_t flags, LD &S, Sink &out) {
   const uint32_t lane = lane_id();
   // the code lengths are dead while a block's symbols are decoded: their
   // bytes hold the window decoder's 64 token slots
@@ -1461,7 +1471,7 @@ __device__ __forceinline__ void inflate_write(Queue &q, uint8_t *ring, uint8_t *
 }
 
 struct Lds2 {
-  Lds dec;                       // the decoding wave's tables, slots and CRC space
+  LdsPair dec;                   // the decoding wave's tables, slots and CRC space
   alignas(16) uint8_t ring[kRingPair];  // the writing wave's output ring
   Queue q;
 };
