@@ -1371,11 +1371,18 @@ __device__ __forceinline__ void lds_release(uint32_t *p, uint32_t v) {
 struct SinkQueue {
   Queue *q;
   uint32_t head;
+  uint32_t tail;  // the writer's last tail this wave read: slots below it are free
   bool ended, stuck;
+  // Hands a message to the writer. The writer's tail is read again only when
+  // the last one read leaves no free slot, and the writer's status is not
+  // polled per batch (finish() returns the writer's first error before any
+  // later decode error), so a batch costs one LDS round trip, the release.
   __device__ __forceinline__ int32_t push(uint32_t lane, uint32_t kind, uint32_t tok, uint32_t nt,
                                           uint64_t at, uint32_t ln) {
     for (uint32_t k = 0;; k++) {  // a free slot
-      if (head - uni(lds_acquire(&q->tail)) < kQueue) break;
+      if (head - tail < kQueue) break;
+      tail = uni(lds_acquire(&q->tail));
+      if (head - tail < kQueue) break;
       if (k >= kSpinLimit) {
         stuck = true;
         return kCorrupt;
@@ -1392,7 +1399,7 @@ struct SinkQueue {
     }
     head++;
     lds_release(&q->head, head);
-    return (int32_t)uni(lds_acquire(reinterpret_cast<uint32_t *>(&q->wstat)));
+    return kOk;
   }
   __device__ __forceinline__ int32_t tokens(uint32_t lane, uint32_t tok, uint32_t nt) {
     if (nt == 0) return (int32_t)uni(lds_acquire(reinterpret_cast<uint32_t *>(&q->wstat)));
@@ -1505,6 +1512,7 @@ inflate_batch2_kernel(const codag_stream_desc *__restrict__ descs, uint64_t n,
       SinkQueue out;
       out.q = &S.q;
       out.head = 0;
+      out.tail = 0;
       out.ended = false;
       out.stuck = false;
       Result r = inflate_stream_t(d.src, d.src_bytes, d.dst, d.dst_cap, flags, S.dec, out);
@@ -1553,6 +1561,7 @@ inflate_chunks2_kernel(const uint8_t *__restrict__ comp, uint64_t comp_bytes,
     SinkQueue sink;
     sink.q = &S.q;
     sink.head = 0;
+    sink.tail = 0;
     sink.ended = false;
     sink.stuck = false;
     Result r = inflate_stream_t(comp + o0, o1 - o0, out + c * chunk_bytes, chunk_bytes, flags, S.dec, sink);
